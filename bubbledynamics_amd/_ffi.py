@@ -1,0 +1,119 @@
+"""ctypes binding of the C ABI in include/bubbledynamics.h.
+
+The shared library is built in-tree by ``bubbledynamics_amd.build``.  Loading
+fails loudly when it is missing: there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libbubbledynamics.so"
+
+BD_OK = 0
+BD_ERR_INVALID = 1
+BD_ERR_UNSTABLE = 2
+BD_ERR_ALLOC = 3
+BD_ERR_HIP = 4
+BD_ERR_NONFINITE = 5
+BD_ERR_UNSUPPORTED = 6
+BD_F32 = 0
+BD_F64 = 1
+BD_BC_MUR = 0
+BD_BC_ZERO = 1
+BD_SRC_CAP = 0
+BD_SRC_ARRAY = 1
+
+_ERRS = {1: "invalid argument", 2: "unstable parameters", 3: "allocation failed",
+         4: "HIP runtime error", 5: "non-finite result", 6: "unsupported option"}
+
+
+class BDError(RuntimeError):
+    def __init__(self, fn: str, code: int):
+        super().__init__(f"{fn} returned {code} ({_ERRS.get(code, 'unknown')})")
+        self.code = code
+
+
+class WesterveltParams(C.Structure):
+    """Mirror of bd_westervelt_params (W:16-52 plus extensions)."""
+    _fields_ = [(n, C.c_double) for n in (
+        "rho", "c_inf", "delta", "beta", "abs_water",
+        "rho_tissue", "c_inf_tissue", "beta_tissue", "abs_tissue",
+        "delta_tissue", "c1", "c2", "tau1", "tau2",
+        "fr", "d", "alpha", "alpha_hole", "p0",
+        "water_tissue",
+        "xy_min", "dxy", "z_min", "dz", "dt")] + [
+        ("full_domain", C.c_int32), ("boundary", C.c_int32), ("source", C.c_int32),
+        ("sponge_cells", C.c_int32), ("sponge_alpha", C.c_double),
+        ("burst_cycles", C.c_double), ("burst_t0", C.c_double),
+        ("arr_nx", C.c_int32), ("arr_ny", C.c_int32),
+        ("arr_x0", C.c_double), ("arr_y0", C.c_double), ("arr_pitch", C.c_double),
+        ("reserved", C.c_double * 8),
+    ]
+
+    def copy(self) -> "WesterveltParams":
+        o = WesterveltParams()
+        C.pointer(o)[0] = self
+        return o
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
+
+
+_VP = C.c_void_p
+_I = C.c_int
+_D = C.c_double
+_PP = C.POINTER(WesterveltParams)
+
+# name -> (restype, argtypes); entries added by the other modules' binding tables
+SIGNATURES: dict[str, tuple] = {
+    "bd_version": (C.c_char_p, []),
+    "bd_device_count": (_I, []),
+    "bd_westervelt_space6": (_I, [_PP, _I] + [_VP] * 12 + [_I, _I, _VP]),
+    "bd_boundary_condition_source": (_I, [_PP, _I, _VP, _D, _I, _I, _VP, _VP, _VP]),
+    "bd_boundary_condition_space6": (_I, [_PP, _I, _VP, _VP, _D, _I, _I, _VP]),
+    "bd_acoustic_power_water": (_I, [_PP, _I, _VP, _VP, _VP, _I, _I, _I, _VP]),
+    "bd_harmonics_accumulate": (_I, [_I, _VP, _VP, _I, _I, _I, _I, _I, _I, _I, _I, _VP]),
+    "bd_harmonics_q": (_I, [_PP, _I, _VP, _VP, _I, _I, _I, _I, _I, _D, _I, _VP]),
+    "bd_westervelt_state_elems": (C.c_size_t, [_I, _I]),
+    "bd_westervelt_plan_create": (_I, [_PP, _I, _I, _I, _I, _I, _I, _I, C.POINTER(_VP)]),
+    "bd_westervelt_plan_bind": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "bd_westervelt_plan_run": (_I, [_VP, _I, _I, _VP]),
+    "bd_westervelt_plan_finalize": (_I, [_VP, _VP]),
+    "bd_westervelt_plan_steps": (_I, [_VP]),
+    "bd_westervelt_plan_slot": (_I, [_VP, _I, _I]),
+    "bd_westervelt_plan_destroy": (_I, [_VP]),
+    "bd_westervelt_plan_interior_only": (_I, [_VP, _VP]),
+    "bd_westervelt_plan_interior_cells": (C.c_longlong, [_VP]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the HIP library; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m bubbledynamics_amd.build` "
+            "(there is no CPU fallback)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(fn: str, rc: int) -> None:
+    if rc != BD_OK:
+        raise BDError(fn, rc)
+
+
+def call(fn: str, *args) -> int:
+    rc = getattr(load(), fn)(*args)
+    check(fn, rc)
+    return rc

@@ -1,0 +1,182 @@
+/*
+ * bubbledynamics.h -- C ABI of the MI355X (gfx950) focused-ultrasound kernels.
+ *
+ * Every entry point takes plain device pointers, plain integers and a HIP
+ * stream passed as `void*` (NULL = the default stream), and returns an int
+ * status (BD_OK = 0).  No C++ or torch types cross this boundary.  Buffers are
+ * owned by the caller; a plan object owns only its scratch (reduction
+ * partials, step counter, time table, checkpoints).
+ *
+ * Drop-in entry points mirror one reference kernel launch each, with the same
+ * grid integers and argument meaning; the reference's __constant__/__device__
+ * symbols become an explicit parameter struct.  Reference files:
+ *   W = Westervelt_relax_fft_Pressure_Energy.cu
+ *   T = Westervelt_Temperature_SquareSpiral.cu
+ *   G = Gilmore_liver_3fr_combo.cu
+ *
+ * Field layout (all 3-D fields): idx = iz*Nxy*Nxy + iy*Nxy + ix, x fastest
+ * (W:69, T:59).  dtype selects the element type of every field argument:
+ * BD_F32 (float) or BD_F64 (double).
+ */
+#ifndef BUBBLEDYNAMICS_H
+#define BUBBLEDYNAMICS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define BD_OK               0
+#define BD_ERR_INVALID      1  /* bad argument (null pointer, size, dtype)      */
+#define BD_ERR_UNSTABLE     2  /* parameters violate the scheme's stability    */
+#define BD_ERR_ALLOC        3  /* device allocation failed or exceeds the cap  */
+#define BD_ERR_HIP          4  /* a HIP runtime call failed                    */
+#define BD_ERR_NONFINITE    5  /* a reduction produced NaN/Inf                  */
+#define BD_ERR_UNSUPPORTED  6  /* option combination not implemented           */
+
+#define BD_F32 0
+#define BD_F64 1
+
+/* ======================================================================== */
+/*  Westervelt acoustics                                                    */
+/* ======================================================================== */
+
+#define BD_BC_MUR   0   /* first-order Mur on 3 layers (W:88-135), outward-ordered */
+#define BD_BC_ZERO  1   /* boundary layers held at p = 0                            */
+
+#define BD_SRC_CAP    0 /* phase-delayed spherical cap with hole (W:62-85)          */
+#define BD_SRC_ARRAY  1 /* square tiling of piston elements, one (A, phi) each      */
+
+/*
+ * Parameters of one Westervelt run.  The first block is the reference's
+ * __constant__ block (W:16-52) field for field.  The second block holds the
+ * extensions; all zeros reproduce the reference.
+ */
+typedef struct bd_westervelt_params {
+    /* water region (z < water_tissue), W:16-20 */
+    double rho, c_inf, delta, beta, abs_water;
+    /* tissue region (z >= water_tissue), W:23-33 */
+    double rho_tissue, c_inf_tissue, beta_tissue, abs_tissue;
+    double delta_tissue, c1, c2, tau1, tau2;
+    /* transducer, W:37-42 */
+    double fr, d, alpha, alpha_hole, p0;
+    double water_tissue;               /* W:44 */
+    /* mesh, W:48-52 */
+    double xy_min, dxy, z_min, dz, dt;
+
+    /* ---- extensions (0 = reference behaviour) ---- */
+    int32_t full_domain;   /* 0: quarter domain, even mirror at ix=0, iy=0 (W:176-200); 1: full domain */
+    int32_t boundary;      /* BD_BC_MUR or BD_BC_ZERO                                               */
+    int32_t source;        /* BD_SRC_CAP or BD_SRC_ARRAY                                            */
+    int32_t sponge_cells;  /* damping layer width at the outer faces (0 = none)                     */
+    double  sponge_alpha;  /* per-step factor exp(-(sponge_alpha*(W-dist))^2) inside the layer      */
+    double  burst_cycles;  /* 0: continuous wave (W:82); >0: rectangular gate of that many cycles   */
+    double  burst_t0;      /* gate opens at delayed time tt - burst_t0 = 0                          */
+    /* BD_SRC_ARRAY: elements tile [arr_x0, arr_x0+arr_nx*arr_pitch) x [arr_y0, ...) on plane iz=2 */
+    int32_t arr_nx, arr_ny;
+    double  arr_x0, arr_y0, arr_pitch;
+    double  reserved[8];
+} bd_westervelt_params;
+
+/* Drop-in for Westervelt_space6<<<>>> (W:139-236): interior update of Pnext
+ * and of the relaxation variables R1, R2 (tissue region only). */
+int bd_westervelt_space6(const bd_westervelt_params* prm, int dtype,
+                         void* Pnext, const void* P, const void* Pm1, const void* Pm2,
+                         const void* Pm3, const void* Pm4,
+                         void* R1, const void* R1m1, const void* R1m2,
+                         void* R2, const void* R2m1, const void* R2m2,
+                         int Nxy, int Nz, void* stream);
+
+/* Drop-in for boundary_condition_source<<<>>> (W:62-85): hard source on
+ * plane iz=2 at time t (t = t^{n+1} as in W:564-566).  For BD_SRC_ARRAY,
+ * elem_amp/elem_phase hold arr_nx*arr_ny doubles each (device memory);
+ * they are ignored (may be NULL) for BD_SRC_CAP. */
+int bd_boundary_condition_source(const bd_westervelt_params* prm, int dtype, void* P,
+                                 double t, int Nxy, int Nz,
+                                 const double* elem_amp, const double* elem_phase,
+                                 void* stream);
+
+/* Drop-in for boundary_condition_space6<<<>>> (W:88-135): first-order Mur
+ * faces (z-min, z-max, x-max, y-max; plus x-min, y-min in a full domain),
+ * evaluated in a defined order: z faces, then x faces, then y faces, each
+ * face innermost layer first, every layer reading its neighbour's new value.
+ * P = Pnext (source plane already set), Pm1 = p^n. */
+int bd_boundary_condition_space6(const bd_westervelt_params* prm, int dtype,
+                                 void* P, const void* Pm1, double t, int Nxy, int Nz,
+                                 void* stream);
+
+/* Drop-in for acoustic_power_water<<<>>> (W:242-265).  last=0 accumulates
+ * Q += ((Pnext-Pm1)/(2dt))^2 dt in the water region; last=1 applies the
+ * final scale 2*abs_water*fr/((2 pi fr)^2 rho c). */
+int bd_acoustic_power_water(const bd_westervelt_params* prm, int dtype, void* Q,
+                            const void* Pnext, const void* Pm1, int Nxy, int Nz,
+                            int last, void* stream);
+
+/* Replaces copy_data + Double_to_Complex + cufftExecC2C + Abs_Complex
+ * (W:268-321, W:740-747) by running DFT accumulators: for harmonic n=1..N_harm,
+ * H[2(n-1)]   += p * cos(2 pi n*periods*i_time/N_time),
+ * H[2(n-1)+1] -= p * sin(2 pi n*periods*i_time/N_time),
+ * on planes iz in [iz_tissue, Nz) (all ix, iy).  H holds 2*N_harm fields of
+ * Nxy*Nxy*(Nz-iz_tissue) elements (same dtype); with kahan=1 it holds 4*N_harm
+ * fields, the last 2*N_harm being Kahan compensations.  These are exactly the
+ * FFT bins k = n*periods the reference reads (W:341). */
+int bd_harmonics_accumulate(int dtype, void* H, const void* Pnext, int Nxy, int Nz,
+                            int iz_tissue, int i_time, int N_time, int periods,
+                            int N_harm, int kahan, void* stream);
+
+/* Drop-in for total_energy_Nharmonics_xyz (W:324-345) over the whole tissue
+ * slab: Q = sum_n 2*(abs_coef*n*fr/1e6) * 2*|X_n/N_time|^2/(rho_t c_t).
+ * abs_coef = 7.2 reproduces W:342.  kahan as in bd_harmonics_accumulate. */
+int bd_harmonics_q(const bd_westervelt_params* prm, int dtype, void* Q, const void* H,
+                   int Nxy, int Nz, int iz_tissue, int N_time, int N_harm,
+                   double abs_coef, int kahan, void* stream);
+
+/* ---- run-level plan: pointer ring + fused kernels + hipGraph ----------- */
+typedef struct bd_westervelt_plan bd_westervelt_plan;
+
+/* State block: 12 fields (6 pressure levels, 3 R1 levels, 3 R2 levels)
+ * laid out back to back, 12*Nxy*Nxy*Nz elements of the plan's dtype. */
+size_t bd_westervelt_state_elems(int Nxy, int Nz);
+
+/* max_steps bounds the per-step outputs (energy, focus series) and the
+ * time table.  focus_* name the cell recorded every step (-1 = none). */
+int bd_westervelt_plan_create(const bd_westervelt_params* prm, int dtype, int Nxy, int Nz,
+                              int max_steps, int focus_ix, int focus_iy, int focus_iz,
+                              bd_westervelt_plan** out);
+/* Binds caller-owned device buffers: state (bd_westervelt_state_elems),
+ * pmax (Nxy*Nxy*Nz, may be NULL), energy (2*max_steps doubles: per step
+ * sum p^2 dV/(rho c^2) [J] and the leapfrog invariant, may be NULL),
+ * focus (max_steps doubles, may be NULL), array element amp/phase (device,
+ * arr_nx*arr_ny doubles each, BD_SRC_ARRAY only). */
+int bd_westervelt_plan_bind(bd_westervelt_plan* plan, void* state, void* pmax, double* energy,
+                            double* focus, const double* elem_amp, const double* elem_phase);
+/* Advances n_steps steps (interior + source + ABC each step).  use_graph=1
+ * replays a captured 6-step hipGraph (the ring period). */
+int bd_westervelt_plan_run(bd_westervelt_plan* plan, int n_steps, int use_graph, void* stream);
+/* Reduces the per-block energy partials of all steps run so far into the
+ * bound energy buffer. */
+int bd_westervelt_plan_finalize(bd_westervelt_plan* plan, void* stream);
+/* Steps run so far, and the ring slot holding p^{n-level} (level 0 = newest)
+ * or R1/R2 levels (which = 0: P, 1: R1, 2: R2). */
+int bd_westervelt_plan_steps(const bd_westervelt_plan* plan);
+int bd_westervelt_plan_slot(const bd_westervelt_plan* plan, int which, int level);
+int bd_westervelt_plan_destroy(bd_westervelt_plan* plan);
+/* Launches only the interior kernel of the next step (same arguments as the
+ * real step, state not advanced) -- used to time that kernel in isolation.
+ * Requires steps < max_steps. */
+int bd_westervelt_plan_interior_only(bd_westervelt_plan* plan, void* stream);
+/* Number of cells the interior kernel updates per launch. */
+long long bd_westervelt_plan_interior_cells(const bd_westervelt_plan* plan);
+
+/* Library metadata */
+const char* bd_version(void);
+int bd_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BUBBLEDYNAMICS_H */
