@@ -64,6 +64,40 @@ __device__ __forceinline__ double block_sum(double v, double* lds /* >= NT/64 */
     return s;
 }
 
+// Raw buffer access: a workgroup-uniform 128-bit resource (base, byte size)
+// plus a 32-bit per-lane byte offset.  Reads past the size return 0 and
+// writes past it are dropped by the hardware.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr int kRsrcWord3 = 0x00020000;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* base, size_t bytes) {
+    const uint32_t n = bytes > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)bytes;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)n, kRsrcWord3);
+}
+
+template <typename T>
+__device__ __forceinline__ T bload(rsrc_t r, uint32_t voff, uint32_t soff = 0);
+template <>
+__device__ __forceinline__ float bload<float>(rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <>
+__device__ __forceinline__ double bload<double>(rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+
+template <typename T>
+__device__ __forceinline__ void bstore(T v, rsrc_t r, uint32_t voff);
+template <>
+__device__ __forceinline__ void bstore<float>(float v, rsrc_t r, uint32_t voff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, 0, 0);
+}
+template <>
+__device__ __forceinline__ void bstore<double>(double v, rsrc_t r, uint32_t voff) {
+    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), r, (int)voff, 0, 0);
+}
+
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 }  // namespace bd

@@ -10,6 +10,8 @@
 // levels p^{n-1..n-4}, the relaxation levels and pmax are streamed once.  Peak
 // pressure, per-step energy and the focus sample are fused reductions.
 #include "common.h"
+#include "wv_march.h"
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -17,39 +19,6 @@
 #include <type_traits>
 
 namespace bd {
-
-// ---------------------------------------------------------------------------
-// coefficients (host-derived from the parameter struct, W:214-230)
-// ---------------------------------------------------------------------------
-template <typename T>
-struct WvCoef {
-    T kxy[2], kz[2], dlt[2], bet[2];   // [0] water, [1] tissue
-    T cdt2;                            // (c_t dt)^2                       W:230
-    T K1, den1, den1b, tau1;           // R1 recurrence                   W:227
-    T K2, den2, den2b, tau2;           // R2 recurrence                   W:228
-    T two_dt3;                         // 2 dt^3                          W:226
-};
-
-struct WvGeom {
-    int Nxy, Nz, xlo, iz_tissue;
-    int ntx, nty, nzc, zchunk, nblocks;
-    int sponge_w, full_domain;
-    double sponge_alpha;
-    double escale[2];                  // dV / (rho c^2), water / tissue
-};
-
-template <typename T>
-struct WvMarchArgs {
-    T* pn; const T* p; const T* pm1; const T* pm2; const T* pm3; const T* pm4;
-    T* r1; const T* r1m1; const T* r1m2; T* r2; const T* r2m1; const T* r2m2;
-    T* pmax;            // nullable
-    double* epart;      // [step][nblocks][2], nullable
-    double* focus;      // [step], nullable
-    const int* d_step;  // step base for graph replay, nullable
-    long long focus_idx;
-    WvGeom g;
-    WvCoef<T> c;
-};
 
 static int first_tissue_plane(const bd_westervelt_params* p, int Nz) {
     for (int iz = 0; iz < Nz; ++iz)
@@ -79,305 +48,129 @@ static WvCoef<T> make_coef(const bd_westervelt_params* p) {
     c.den2b = (T)(2.0 * dt + 3.0 * p->tau2);
     c.tau2 = (T)p->tau2;
     c.two_dt3 = (T)(2.0 * std::pow(dt, 3));
+    const double two_dt3 = 2.0 * std::pow(dt, 3);
+    const double ct3 = std::pow(p->c_inf_tissue, 3);
+    c.a1 = (T)((2.0 * dt * 2.0 * p->c1 * p->tau1) / (two_dt3 * (2.0 * dt + 3.0 * p->tau1) * ct3));
+    c.b1 = (T)(p->tau1 / (2.0 * dt + 3.0 * p->tau1));
+    c.a2 = (T)((2.0 * dt * 2.0 * p->c2 * p->tau2) / (two_dt3 * (2.0 * dt + 3.0 * p->tau2) * ct3));
+    c.b2 = (T)(p->tau2 / (2.0 * dt + 3.0 * p->tau2));
     return c;
-}
-
-// ---------------------------------------------------------------------------
-// interior z-march
-// ---------------------------------------------------------------------------
-template <typename T, int TX, int TY>
-__global__ void __launch_bounds__(TX * TY)
-wv_march(WvMarchArgs<T> a, int step_arg) {
-    constexpr int NT = TX * TY;
-    __shared__ T s[2][TY + 6][TX + 6];
-    __shared__ double red[NT / kWave];
-
-    const int step = a.d_step ? (*a.d_step + step_arg) : step_arg;
-    const int lb = xcd_remap((int)blockIdx.x, a.g.nblocks);
-    const int zc = lb % a.g.nzc, t2 = lb / a.g.nzc;
-    const int bx = t2 % a.g.ntx, by = t2 / a.g.ntx;
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const int N = a.g.Nxy;
-    const size_t plane = (size_t)N * N;
-    const int ix = a.g.xlo + bx * TX + tx, iy = a.g.xlo + by * TY + ty;
-    const bool in_arr = ix < N && iy < N;
-    const bool active = ix <= N - 4 && iy <= N - 4;
-    const size_t cidx = (size_t)iy * N + ix;
-
-    // halo sources of this thread (even mirror at index < 0, W:176-200)
-    int gxl = a.g.xlo + bx * TX - 3 + tx; if (gxl < 0) gxl = -gxl;
-    int gyb = a.g.xlo + by * TY - 3 + ty; if (gyb < 0) gyb = -gyb;
-    const bool vL = tx < 3 && iy < N && gxl < N;
-    const bool vR = tx >= TX - 3 && iy < N && ix + 3 < N;
-    const bool vB = ty < 3 && ix < N && gyb < N;
-    const bool vT = ty >= TY - 3 && ix < N && iy + 3 < N;
-    const size_t offL = (size_t)iy * N + gxl, offR = cidx + 3;
-    const size_t offB = (size_t)gyb * N + ix, offT = cidx + 3 * (size_t)N;
-
-    const int z0 = 3 + zc * a.g.zchunk;
-    const int z1 = min(z0 + a.g.zchunk, a.g.Nz - 3);
-
-    T q[7];
-#pragma unroll
-    for (int k = 0; k < 7; ++k) q[k] = T(0);
-    if (in_arr) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) q[k] = a.p[(size_t)(z0 - 3 + k) * plane + cidx];
-    }
-    double e0 = 0.0, e1 = 0.0;
-    const int W = a.g.sponge_w;
-
-    for (int z = z0; z < z1; ++z) {
-        const int b = (z - z0) & 1;
-        const size_t zo = (size_t)z * plane;
-        if (in_arr) q[6] = a.p[zo + 3 * plane + cidx];
-        s[b][ty + 3][tx + 3] = q[3];
-        if (tx < 3) s[b][ty + 3][tx] = vL ? a.p[zo + offL] : T(0);
-        if (tx >= TX - 3) s[b][ty + 3][tx + 6] = vR ? a.p[zo + offR] : T(0);
-        if (ty < 3) s[b][ty][tx + 3] = vB ? a.p[zo + offB] : T(0);
-        if (ty >= TY - 3) s[b][ty + 6][tx + 3] = vT ? a.p[zo + offT] : T(0);
-
-        const size_t idx = zo + cidx;
-        const int reg = z >= a.g.iz_tissue ? 1 : 0;
-        T pm1 = T(0), pm2 = T(0), pm3 = T(0), pm4 = T(0);
-        T r1m1 = T(0), r1m2 = T(0), r2m1 = T(0), r2m2 = T(0), pmx = T(0);
-        if (active) {
-            pm1 = a.pm1[idx]; pm2 = a.pm2[idx]; pm3 = a.pm3[idx]; pm4 = a.pm4[idx];
-            if (reg) { r1m1 = a.r1m1[idx]; r1m2 = a.r1m2[idx]; r2m1 = a.r2m1[idx]; r2m2 = a.r2m2[idx]; }
-            if (a.pmax) pmx = a.pmax[idx];
-        }
-        __syncthreads();
-        if (active) {
-            const T pc = q[3];
-            const T* row = &s[b][ty + 3][tx + 3];
-            const T Dxx = (T(-490.0) * pc + T(270.0) * (row[1] + row[-1]) - T(27.0) * (row[2] + row[-2]) +
-                           T(2.0) * (row[3] + row[-3])) / T(180.0);
-            const T Dyy = (T(-490.0) * pc + T(270.0) * (s[b][ty + 4][tx + 3] + s[b][ty + 2][tx + 3]) -
-                           T(27.0) * (s[b][ty + 5][tx + 3] + s[b][ty + 1][tx + 3]) +
-                           T(2.0) * (s[b][ty + 6][tx + 3] + s[b][ty][tx + 3])) / T(180.0);
-            const T Dzz = (T(-490.0) * pc + T(270.0) * (q[4] + q[2]) - T(27.0) * (q[5] + q[1]) +
-                           T(2.0) * (q[6] + q[0])) / T(180.0);
-            T Dttt, N12;
-            if constexpr (std::is_same<T, float>::value) {
-                // Same polynomials written on first differences of the history,
-                // which FP32 forms exactly (Sterbenz); cuts cancellation ~10x.
-                const T d1 = pc - pm1, d2 = pm1 - pm2, d3 = pm2 - pm3, d4 = pm3 - pm4;
-                Dttt = T(5.0) * d1 - T(13.0) * d2 + T(11.0) * d3 - T(3.0) * d4;
-                const T A = T(3.0) * d1 - d2;
-                N12 = (A * A) / T(4.0) + pc * (T(2.0) * d1 - T(3.0) * d2 + d3);
-            } else {
-                // W:207-209, reference expression order
-                Dttt = T(5.0) * pc - T(18.0) * pm1 + T(24.0) * pm2 - T(14.0) * pm3 + T(3.0) * pm4;
-                const T A = T(3.0) * pc - T(4.0) * pm1 + pm2;
-                N12 = (A * A) / T(4.0) + pc * (T(2.0) * pc - T(5.0) * pm1 + T(4.0) * pm2 - pm3);
-            }
-            T sumR = T(0);
-            T r1 = T(0), r2 = T(0);
-            if (reg) {  // W:226-230
-                const T Dttt_2dt = Dttt / a.c.two_dt3;
-                r1 = (a.c.K1 * Dttt_2dt / a.c.den1) + ((a.c.tau1 * (T(4.0) * r1m1 - r1m2)) / a.c.den1b);
-                r2 = (a.c.K2 * Dttt_2dt / a.c.den2) + ((a.c.tau2 * (T(4.0) * r2m1 - r2m2)) / a.c.den2b);
-                sumR = (r1 + r2) * a.c.cdt2;
-            }
-            const T lap = a.c.kxy[reg] * (Dxx + Dyy) + a.c.kz[reg] * Dzz;
-            T pn;
-            if constexpr (std::is_same<T, float>::value) {
-                const T inc = (pc - pm1) + lap + a.c.dlt[reg] * Dttt + a.c.bet[reg] * N12 + sumR;
-                pn = pc + inc;
-            } else {  // W:234
-                pn = T(2.0) * pc - pm1 + a.c.kxy[reg] * (Dxx + Dyy) + a.c.kz[reg] * Dzz +
-                     a.c.dlt[reg] * Dttt + a.c.bet[reg] * N12 + sumR;
-            }
-            if (W > 0) {
-                int dx = N - 1 - ix, dy = N - 1 - iy, dz = a.g.Nz - 1 - z;
-                if (a.g.full_domain) { dx = min(dx, ix); dy = min(dy, iy); }
-                T sp = T(0);
-                if (dx < W) { const T u = T(a.g.sponge_alpha) * T(W - dx); sp += u * u; }
-                if (dy < W) { const T u = T(a.g.sponge_alpha) * T(W - dy); sp += u * u; }
-                if (dz < W) { const T u = T(a.g.sponge_alpha) * T(W - dz); sp += u * u; }
-                if (sp != T(0)) pn = exp(-sp) * pn;
-            }
-            a.pn[idx] = pn;
-            if (reg) { a.r1[idx] = r1; a.r2[idx] = r2; }
-            if (a.pmax && pn > pmx) a.pmax[idx] = pn;
-            if ((long long)idx == a.focus_idx && a.focus) a.focus[step] = (double)pn;
-            const double dpn = (double)pn, dd = (double)pn - (double)pc;
-            e0 += dpn * dpn * a.g.escale[reg];
-            e1 += dd * dd - dpn * (double)lap;
-        }
-#pragma unroll
-        for (int k = 0; k < 6; ++k) q[k] = q[k + 1];
-    }
-    if (a.epart) {
-        const double s0 = block_sum<NT>(e0, red);
-        const double s1 = block_sum<NT>(e1, red);
-        if (threadIdx.x == 0) {
-            double* o = a.epart + ((size_t)step * a.g.nblocks + blockIdx.x) * 2;
-            o[0] = s0;
-            o[1] = s1;
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------
 // source plane + boundary layers (one launch, no inter-thread dependency)
 // ---------------------------------------------------------------------------
-struct WvSrc {
-    int kind;                 // BD_SRC_*
-    double xy_min, dxy, d, rmax, rmin, c_inf, fr, p0, burst_cycles, burst_t0;
-    int arr_nx, arr_ny;
-    double arr_x0, arr_y0, arr_pitch;
-    const double* amp; const double* phase;
-};
-
-__device__ __forceinline__ double source_value(const WvSrc& s, int ix, int iy, double t) {
-    const double X = s.xy_min + ix * s.dxy;
-    const double Y = s.xy_min + iy * s.dxy;
-    if (s.kind == BD_SRC_ARRAY) {
-        const double fx = floor((X - s.arr_x0) / s.arr_pitch);
-        const double fy = floor((Y - s.arr_y0) / s.arr_pitch);
-        if (fx < 0 || fy < 0 || fx >= s.arr_nx || fy >= s.arr_ny) return 0.0;
-        const int e = (int)fy * s.arr_nx + (int)fx;
-        double tb = t;
-        if (s.burst_cycles > 0) {
-            tb = t - s.burst_t0;
-            if (tb < 0.0 || tb * s.fr >= s.burst_cycles) return 0.0;
-        }
-        return s.amp[e] * sin(2 * M_PI * s.fr * tb + s.phase[e]);
-    }
-    // W:72-83
-    const double r = sqrt(X * X + Y * Y);
-    if (r <= s.rmax && r >= s.rmin) {
-        const double rd = r / s.d;
-        const double var = sqrt(1.0 + rd * rd);
-        double tt = t + (s.d * (var - 1.0) / s.c_inf);
-        if (s.burst_cycles > 0) {
-            tt = tt - s.burst_t0;
-            if (tt < 0.0 || tt * s.fr >= s.burst_cycles) return 0.0;
-        }
-        return s.p0 * sin(2 * M_PI * s.fr * tt) / var;
-    }
-    return 0.0;
-}
-
 template <typename T>
-struct WvBndArgs {
+struct WvFaceArgs {
     T* P; const T* Pm1; T* pmax;
-    const int* d_step; const double* d_time;  // t = d_time[step+1] when d_step != null
-    double t;                                 // otherwise
-    int Nxy, Nz, lo, iz_tissue, boundary, fuse_source;
+    int Nxy, Nz, lo, iz_tissue, boundary;
     double kz_w, kz_t, kxy_w, kxy_t;          // Courant numbers c dt / h (W:100, W:111, W:117-119)
-    WvSrc src;
 };
 
-// Mur update in the expression order of W:104.
+// Z faces of the interior columns (W:101-113), innermost layer first; the
+// source plane iz=2 must already hold p^{n+1}.  Used by the drop-in boundary
+// entry; the plan fuses this work into the interior kernel.
 template <typename T>
-__device__ __forceinline__ T mur(T pm1_nb, T p_nb, T pm1_self, double k) {
-    return pm1_nb + ((p_nb - pm1_self) * T(k - 1.0) / T(k + 1.0));
+__global__ void __launch_bounds__(256) wv_zfaces(WvFaceArgs<T> a) {
+    const int N = a.Nxy, lo = a.lo, nin = N - 3 - lo;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nin * nin) return;
+    const int ix = lo + g % nin, iy = lo + g / nin;
+    const size_t plane = (size_t)N * N, c = (size_t)iy * N + ix;
+    T v1, v0, u[3];
+    if (a.boundary == BD_BC_ZERO) {
+        v1 = v0 = u[0] = u[1] = u[2] = T(0);
+    } else {
+        const T s2 = a.P[2 * plane + c];
+        const T m0 = a.Pm1[c], m1 = a.Pm1[plane + c], m2 = a.Pm1[2 * plane + c];
+        v1 = mur<T>(m2, s2, m1, a.kz_w);
+        v0 = mur<T>(m1, v1, m0, a.kz_w);
+        const size_t zt = (size_t)(a.Nz - 4) * plane + c;
+        const T b = a.P[zt];
+        const T n0 = a.Pm1[zt], n1 = a.Pm1[zt + plane], n2 = a.Pm1[zt + 2 * plane], n3 = a.Pm1[zt + 3 * plane];
+        u[0] = mur<T>(n0, b, n1, a.kz_t);
+        u[1] = mur<T>(n1, u[0], n2, a.kz_t);
+        u[2] = mur<T>(n2, u[1], n3, a.kz_t);
+    }
+    auto put = [&](size_t i, T v) { a.P[i] = v; if (a.pmax && v > a.pmax[i]) a.pmax[i] = v; };
+    put(plane + c, v1);
+    put(c, v0);
+    for (int k = 0; k < 3; ++k) put((size_t)(a.Nz - 3 + k) * plane + c, u[k]);
 }
 
+// X faces then Y faces (W:121-133), innermost layer first, every layer
+// reading its inner neighbour's new value; the Z faces are already in P.
+// Threads [0, nX) own one (iy, iz) row of the x layers (iy inside); threads
+// [nX, nX+nY) own one (ix, iz) column of the y layers (all ix).  A y-layer
+// thread in an x-layer column recomputes the x-layer value it starts from.
 template <typename T>
-struct Bnd {
-    const WvBndArgs<T>& a;
-    double t;
-    size_t plane;
-    __device__ Bnd(const WvBndArgs<T>& a_, double t_) : a(a_), t(t_), plane((size_t)a_.Nxy * a_.Nxy) {}
-    __device__ size_t at(int ix, int iy, int iz) const { return (size_t)iz * plane + (size_t)iy * a.Nxy + ix; }
-    __device__ double kxy(int iz) const { return iz < a.iz_tissue ? a.kxy_w : a.kxy_t; }
-    __device__ T src(int ix, int iy) const {
-        return a.fuse_source ? (T)source_value(a.src, ix, iy, t) : a.P[at(ix, iy, 2)];
-    }
-    // New value of a cell in an interior column (x, y inside the update range).
-    __device__ T val0(int ix, int iy, int iz) const {
-        if (iz >= 3 && iz <= a.Nz - 4) return a.P[at(ix, iy, iz)];
-        if (iz == 2) return src(ix, iy);
-        if (a.boundary == BD_BC_ZERO) return T(0);
-        if (iz <= 1) {
-            T v = mur<T>(a.Pm1[at(ix, iy, 2)], src(ix, iy), a.Pm1[at(ix, iy, 1)], a.kz_w);
-            if (iz == 1) return v;
-            return mur<T>(a.Pm1[at(ix, iy, 1)], v, a.Pm1[at(ix, iy, 0)], a.kz_w);
+__global__ void __launch_bounds__(256) wv_xyfaces(WvFaceArgs<T> a) {
+    const int N = a.Nxy, Nz = a.Nz, lo = a.lo, nin = N - 3 - lo;
+    const long long nX = (long long)nin * Nz, nY = (long long)N * Nz;
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nX + nY) return;
+    const size_t plane = (size_t)N * N;
+    const bool zero = a.boundary == BD_BC_ZERO;
+    auto put = [&](size_t i, T v) { a.P[i] = v; if (a.pmax && v > a.pmax[i]) a.pmax[i] = v; };
+    if (g < nX) {
+        const int iy = lo + (int)(g % nin), iz = (int)(g / nin);
+        const double k = iz < a.iz_tissue ? a.kxy_w : a.kxy_t;
+        const size_t r0 = (size_t)iz * plane + (size_t)iy * N;
+        {
+            const T b = a.P[r0 + N - 4];
+            const T m0 = a.Pm1[r0 + N - 4], m1 = a.Pm1[r0 + N - 3], m2 = a.Pm1[r0 + N - 2], m3 = a.Pm1[r0 + N - 1];
+            const T x0 = zero ? T(0) : mur<T>(m0, b, m1, k);
+            const T x1 = zero ? T(0) : mur<T>(m1, x0, m2, k);
+            const T x2 = zero ? T(0) : mur<T>(m2, x1, m3, k);
+            put(r0 + N - 3, x0); put(r0 + N - 2, x1); put(r0 + N - 1, x2);
         }
-        T v = a.P[at(ix, iy, a.Nz - 4)];
-        for (int z = a.Nz - 3; z <= iz; ++z) v = mur<T>(a.Pm1[at(ix, iy, z - 1)], v, a.Pm1[at(ix, iy, z)], a.kz_t);
-        return v;
+        if (lo) {
+            const T b = a.P[r0 + 3];
+            const T m0 = a.Pm1[r0 + 3], m1 = a.Pm1[r0 + 2], m2 = a.Pm1[r0 + 1], m3 = a.Pm1[r0];
+            const T x0 = zero ? T(0) : mur<T>(m0, b, m1, k);
+            const T x1 = zero ? T(0) : mur<T>(m1, x0, m2, k);
+            const T x2 = zero ? T(0) : mur<T>(m2, x1, m3, k);
+            put(r0 + 2, x0); put(r0 + 1, x1); put(r0, x2);
+        }
+        return;
     }
-    // New value at x-layer cell (ix in {0,1,2} or {N-3..N-1}), iy inside.
-    __device__ T valX(int ix, int iy, int iz) const {
-        if (a.boundary == BD_BC_ZERO) return T(0);
-        const double k = kxy(iz);
-        if (ix >= a.Nxy - 3) {
-            T v = val0(a.Nxy - 4, iy, iz);
-            for (int x = a.Nxy - 3; x <= ix; ++x) v = mur<T>(a.Pm1[at(x - 1, iy, iz)], v, a.Pm1[at(x, iy, iz)], k);
+    const long long h = g - nX;
+    const int ix = (int)(h % N), iz = (int)(h / N);
+    const double k = iz < a.iz_tissue ? a.kxy_w : a.kxy_t;
+    const size_t zo = (size_t)iz * plane;
+    // value at (ix, iyb, iz) after the x faces, for iyb an inside row
+    auto after_x = [&](int iyb) -> T {
+        const size_t r0 = zo + (size_t)iyb * N;
+        if (ix >= lo && ix <= N - 4) return a.P[r0 + ix];
+        if (zero) return T(0);
+        if (ix >= N - 3) {
+            T v = a.P[r0 + N - 4];
+            for (int x = N - 3; x <= ix; ++x) v = mur<T>(a.Pm1[r0 + x - 1], v, a.Pm1[r0 + x], k);
             return v;
         }
-        T v = val0(3, iy, iz);
-        for (int x = 2; x >= ix; --x) v = mur<T>(a.Pm1[at(x + 1, iy, iz)], v, a.Pm1[at(x, iy, iz)], k);
+        T v = a.P[r0 + 3];
+        for (int x = 2; x >= ix; --x) v = mur<T>(a.Pm1[r0 + x + 1], v, a.Pm1[r0 + x], k);
         return v;
+    };
+    {
+        const T b = after_x(N - 4);
+        const size_t c = zo + ix;
+        const T m0 = a.Pm1[c + (size_t)(N - 4) * N], m1 = a.Pm1[c + (size_t)(N - 3) * N];
+        const T m2 = a.Pm1[c + (size_t)(N - 2) * N], m3 = a.Pm1[c + (size_t)(N - 1) * N];
+        const T y0 = zero ? T(0) : mur<T>(m0, b, m1, k);
+        const T y1 = zero ? T(0) : mur<T>(m1, y0, m2, k);
+        const T y2 = zero ? T(0) : mur<T>(m2, y1, m3, k);
+        put(c + (size_t)(N - 3) * N, y0); put(c + (size_t)(N - 2) * N, y1); put(c + (size_t)(N - 1) * N, y2);
     }
-    __device__ T val1(int ix, int iy, int iz) const {
-        const bool xl = ix >= a.Nxy - 3 || ix < a.lo;
-        return xl ? valX(ix, iy, iz) : val0(ix, iy, iz);
-    }
-    __device__ void put(int ix, int iy, int iz, T v) const {
-        const size_t i = at(ix, iy, iz);
-        a.P[i] = v;
-        if (a.pmax && v > a.pmax[i]) a.pmax[i] = v;
-    }
-};
-
-template <typename T>
-__global__ void __launch_bounds__(256) wv_boundary(WvBndArgs<T> a) {
-    const double t = a.d_step ? a.d_time[*a.d_step + 1] : a.t;
-    Bnd<T> B(a, t);
-    const int N = a.Nxy, Nz = a.Nz, lo = a.lo;
-    const int nin = N - 3 - lo;                    // interior columns per axis
-    const long long nZ = (long long)nin * nin;
-    const long long nX = (long long)nin * Nz;
-    const long long nY = (long long)N * Nz;
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < nZ) {
-        const int ix = lo + (int)(g % nin), iy = lo + (int)(g / nin);
-        if (a.fuse_source) B.put(ix, iy, 2, B.src(ix, iy));
-        B.put(ix, iy, 1, B.val0(ix, iy, 1));
-        B.put(ix, iy, 0, B.val0(ix, iy, 0));
-        for (int z = Nz - 3; z <= Nz - 1; ++z) B.put(ix, iy, z, B.val0(ix, iy, z));
-        return;
-    }
-    if (g < nZ + nX) {
-        const long long h = g - nZ;
-        const int iy = lo + (int)(h % nin), iz = (int)(h / nin);
-        // x-max layers, innermost first
-        T v = B.val0(N - 4, iy, iz);
-        const double k = B.kxy(iz);
-        for (int x = N - 3; x <= N - 1; ++x) {
-            v = (a.boundary == BD_BC_ZERO) ? T(0) : mur<T>(a.Pm1[B.at(x - 1, iy, iz)], v, a.Pm1[B.at(x, iy, iz)], k);
-            B.put(x, iy, iz, v);
-        }
-        if (lo) {
-            v = B.val0(3, iy, iz);
-            for (int x = 2; x >= 0; --x) {
-                v = (a.boundary == BD_BC_ZERO) ? T(0) : mur<T>(a.Pm1[B.at(x + 1, iy, iz)], v, a.Pm1[B.at(x, iy, iz)], k);
-                B.put(x, iy, iz, v);
-            }
-        }
-        return;
-    }
-    if (g < nZ + nX + nY) {
-        const long long h = g - nZ - nX;
-        const int ix = (int)(h % N), iz = (int)(h / N);
-        const double k = B.kxy(iz);
-        T v = B.val1(ix, N - 4, iz);
-        for (int y = N - 3; y <= N - 1; ++y) {
-            v = (a.boundary == BD_BC_ZERO) ? T(0) : mur<T>(a.Pm1[B.at(ix, y - 1, iz)], v, a.Pm1[B.at(ix, y, iz)], k);
-            B.put(ix, y, iz, v);
-        }
-        if (lo) {
-            v = B.val1(ix, 3, iz);
-            for (int y = 2; y >= 0; --y) {
-                v = (a.boundary == BD_BC_ZERO) ? T(0) : mur<T>(a.Pm1[B.at(ix, y + 1, iz)], v, a.Pm1[B.at(ix, y, iz)], k);
-                B.put(ix, y, iz, v);
-            }
-        }
+    if (lo) {
+        const T b = after_x(3);
+        const size_t c = zo + ix;
+        const T m0 = a.Pm1[c + 3 * (size_t)N], m1 = a.Pm1[c + 2 * (size_t)N];
+        const T m2 = a.Pm1[c + (size_t)N], m3 = a.Pm1[c];
+        const T y0 = zero ? T(0) : mur<T>(m0, b, m1, k);
+        const T y1 = zero ? T(0) : mur<T>(m1, y0, m2, k);
+        const T y2 = zero ? T(0) : mur<T>(m2, y1, m3, k);
+        put(c + 2 * (size_t)N, y0); put(c + (size_t)N, y1); put(c, y2);
     }
 }
 
@@ -480,7 +273,23 @@ __global__ void energy_finalize_kernel(const double* epart, double* energy, int 
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
-constexpr int kTX = 64, kTY = 4;
+constexpr int kDefaultVariant = 2;
+constexpr int kDefaultBlocks = 2048;
+
+// Variant / target block count for the interior kernel; BD_WV_VARIANT and
+// BD_WV_BLOCKS override the defaults (used by the tile-shape sweep).
+static void launch_shape(int* variant, int* blocks) {
+    *variant = kDefaultVariant;
+    *blocks = kDefaultBlocks;
+    if (const char* v = std::getenv("BD_WV_VARIANT")) {
+        const int x = std::atoi(v);
+        if (x >= 0 && x < kWvNumVariants) *variant = x;
+    }
+    if (const char* v = std::getenv("BD_WV_BLOCKS")) {
+        const int x = std::atoi(v);
+        if (x > 0) *blocks = x;
+    }
+}
 
 static int check_params(const bd_westervelt_params* p, int dtype, int Nxy, int Nz) {
     if (!p || (dtype != BD_F32 && dtype != BD_F64)) return BD_ERR_INVALID;
@@ -497,14 +306,15 @@ static int check_params(const bd_westervelt_params* p, int dtype, int Nxy, int N
     return BD_OK;
 }
 
-static WvGeom make_geom(const bd_westervelt_params* p, int Nxy, int Nz, int target_blocks) {
+static WvGeom make_geom(const bd_westervelt_params* p, int Nxy, int Nz, int variant, int target_blocks) {
     WvGeom g;
+    const WvVariant v = kWvVariants[variant];
     g.Nxy = Nxy; g.Nz = Nz;
     g.xlo = p->full_domain ? 3 : 0;
     g.iz_tissue = first_tissue_plane(p, Nz);
     const int nin = Nxy - 3 - g.xlo;
-    g.ntx = (nin + kTX - 1) / kTX;
-    g.nty = (nin + kTY - 1) / kTY;
+    g.ntx = (nin + v.tx - 1) / v.tx;
+    g.nty = (nin + v.ty * v.ry - 1) / (v.ty * v.ry);
     const int nzin = Nz - 6;
     int nzc = std::max(1, target_blocks / (g.ntx * g.nty));
     int zchunk = (nzin + nzc - 1) / nzc;
@@ -535,39 +345,72 @@ static WvSrc make_src(const bd_westervelt_params* p, const double* amp, const do
     return s;
 }
 
+template <typename T, int V>
+static void launch_variant(const WvMarchArgs<T>& a, int step_arg, hipStream_t st) {
+    constexpr WvVariant v = kWvVariants[V];
+    hipLaunchKernelGGL((wv_march<T, v.tx, v.ty, v.ry, (bool)v.pf>), dim3(a.g.nblocks), dim3(v.tx * v.ty), 0,
+                       st, a, step_arg);
+}
+
 template <typename T>
-static int launch_march(const WvMarchArgs<T>& a, int step_arg, hipStream_t st) {
-    hipLaunchKernelGGL((wv_march<T, kTX, kTY>), dim3(a.g.nblocks), dim3(kTX * kTY), 0, st, a, step_arg);
+static int launch_march(const WvMarchArgs<T>& a, int variant, int step_arg, hipStream_t st) {
+    switch (variant) {
+        case 0: launch_variant<T, 0>(a, step_arg, st); break;
+        case 1: launch_variant<T, 1>(a, step_arg, st); break;
+        case 2: launch_variant<T, 2>(a, step_arg, st); break;
+        case 3: launch_variant<T, 3>(a, step_arg, st); break;
+        case 4: launch_variant<T, 4>(a, step_arg, st); break;
+        case 5: launch_variant<T, 5>(a, step_arg, st); break;
+        case 6: launch_variant<T, 6>(a, step_arg, st); break;
+        case 7: launch_variant<T, 7>(a, step_arg, st); break;
+        default: return BD_ERR_INVALID;
+    }
     BD_LAUNCH_CHECK();
     return BD_OK;
 }
 
 template <typename T>
-static WvBndArgs<T> make_bnd(const bd_westervelt_params* p, int Nxy, int Nz, T* P, const T* Pm1,
-                             T* pmax, const double* amp, const double* phase) {
-    WvBndArgs<T> b;
-    b.P = P; b.Pm1 = Pm1; b.pmax = pmax;
-    b.d_step = nullptr; b.d_time = nullptr; b.t = 0.0;
-    b.Nxy = Nxy; b.Nz = Nz; b.lo = p->full_domain ? 3 : 0;
-    b.iz_tissue = first_tissue_plane(p, Nz);
-    b.boundary = p->boundary;
-    b.fuse_source = 1;
-    b.kz_w = p->c_inf * p->dt / p->dz;
-    b.kz_t = p->c_inf_tissue * p->dt / p->dz;
-    b.kxy_w = p->c_inf * p->dt / p->dxy;
-    b.kxy_t = p->c_inf_tissue * p->dt / p->dxy;
-    b.src = make_src(p, amp, phase);
-    return b;
+static WvFaceArgs<T> make_faces(const bd_westervelt_params* p, int Nxy, int Nz, T* P, const T* Pm1, T* pmax) {
+    WvFaceArgs<T> f;
+    f.P = P; f.Pm1 = Pm1; f.pmax = pmax;
+    f.Nxy = Nxy; f.Nz = Nz; f.lo = p->full_domain ? 3 : 0;
+    f.iz_tissue = first_tissue_plane(p, Nz);
+    f.boundary = p->boundary;
+    f.kz_w = p->c_inf * p->dt / p->dz;
+    f.kz_t = p->c_inf_tissue * p->dt / p->dz;
+    f.kxy_w = p->c_inf * p->dt / p->dxy;
+    f.kxy_t = p->c_inf_tissue * p->dt / p->dxy;
+    return f;
 }
 
 template <typename T>
-static int launch_bnd(const WvBndArgs<T>& b, hipStream_t st) {
-    const int nin = b.Nxy - 3 - b.lo;
-    const long long n = (long long)nin * nin + (long long)nin * b.Nz + (long long)b.Nxy * b.Nz;
-    const int blocks = (int)((n + 255) / 256);
-    hipLaunchKernelGGL((wv_boundary<T>), dim3(blocks), dim3(256), 0, st, b);
+static int launch_xyfaces(const WvFaceArgs<T>& f, hipStream_t st) {
+    const int nin = f.Nxy - 3 - f.lo;
+    const long long n = (long long)nin * f.Nz + (long long)f.Nxy * f.Nz;
+    hipLaunchKernelGGL((wv_xyfaces<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, f);
     BD_LAUNCH_CHECK();
     return BD_OK;
+}
+
+template <typename T>
+static int launch_zfaces(const WvFaceArgs<T>& f, hipStream_t st) {
+    const int nin = f.Nxy - 3 - f.lo;
+    hipLaunchKernelGGL((wv_zfaces<T>), dim3((nin * nin + 255) / 256), dim3(256), 0, st, f);
+    BD_LAUNCH_CHECK();
+    return BD_OK;
+}
+
+// Fills the march arguments that fuse the source plane and the z faces.
+template <typename T>
+static void set_fused_z(WvMarchArgs<T>& a, const bd_westervelt_params* p, int on, const double* amp,
+                        const double* phase) {
+    a.fuse_z = on;
+    a.boundary = p->boundary;
+    a.kz_w = p->c_inf * p->dt / p->dz;
+    a.kz_t = p->c_inf_tissue * p->dt / p->dz;
+    a.d_time = nullptr;
+    a.t_next = 0.0;
+    a.src = make_src(p, amp, phase);
 }
 
 }  // namespace bd
@@ -588,9 +431,12 @@ static int space6_t(const bd_westervelt_params* prm, void* Pnext, const void* P,
     a.r1 = (T*)R1; a.r1m1 = (const T*)R1m1; a.r1m2 = (const T*)R1m2;
     a.r2 = (T*)R2; a.r2m1 = (const T*)R2m1; a.r2m2 = (const T*)R2m2;
     a.pmax = nullptr; a.epart = nullptr; a.focus = nullptr; a.d_step = nullptr; a.focus_idx = -1;
-    a.g = make_geom(prm, Nxy, Nz, 2048);
+    int variant, blocks;
+    launch_shape(&variant, &blocks);
+    a.g = make_geom(prm, Nxy, Nz, variant, blocks);
     a.c = make_coef<T>(prm);
-    return launch_march<T>(a, 0, st);
+    set_fused_z<T>(a, prm, 0, nullptr, nullptr);
+    return launch_march<T>(a, variant, 0, st);
 }
 
 extern "C" int bd_westervelt_space6(const bd_westervelt_params* prm, int dtype, void* Pnext,
@@ -632,9 +478,11 @@ extern "C" int bd_boundary_condition_source(const bd_westervelt_params* prm, int
 template <typename T>
 static int abc_t(const bd_westervelt_params* prm, void* P, const void* Pm1, double t, int Nxy, int Nz,
                  hipStream_t st) {
-    WvBndArgs<T> b = make_bnd<T>(prm, Nxy, Nz, (T*)P, (const T*)Pm1, nullptr, nullptr, nullptr);
-    b.t = t; b.fuse_source = 0;
-    return launch_bnd<T>(b, st);
+    (void)t;
+    const WvFaceArgs<T> f = make_faces<T>(prm, Nxy, Nz, (T*)P, (const T*)Pm1, nullptr);
+    int rc = launch_zfaces<T>(f, st);
+    if (rc) return rc;
+    return launch_xyfaces<T>(f, st);
 }
 
 extern "C" int bd_boundary_condition_space6(const bd_westervelt_params* prm, int dtype, void* P,
@@ -721,6 +569,7 @@ struct bd_westervelt_plan {
     int dtype, Nxy, Nz, max_steps;
     long long focus_idx;
     WvGeom g;
+    int variant = 0;
     void* state = nullptr; void* pmax = nullptr;
     double* energy = nullptr; double* focus = nullptr;
     const double* amp = nullptr; const double* phase = nullptr;
@@ -749,7 +598,9 @@ extern "C" int bd_westervelt_plan_create(const bd_westervelt_params* prm, int dt
     if (max_steps <= 0) return BD_ERR_INVALID;
     auto* pl = new bd_westervelt_plan();
     pl->prm = *prm; pl->dtype = dtype; pl->Nxy = Nxy; pl->Nz = Nz; pl->max_steps = max_steps;
-    pl->g = make_geom(prm, Nxy, Nz, 2048);
+    int blocks = 0;
+    launch_shape(&pl->variant, &blocks);
+    pl->g = make_geom(prm, Nxy, Nz, pl->variant, blocks);
     const int lo = prm->full_domain ? 3 : 0;
     pl->focus_idx = -1;
     if (fx >= 0) {
@@ -807,20 +658,13 @@ static int plan_step(bd_westervelt_plan* pl, int n, const int* d_step, int step_
     a.focus_idx = pl->focus_idx;
     a.g = pl->g;
     a.c = make_coef<T>(&pl->prm);
-    int rc = launch_march<T>(a, step_arg, st);
+    set_fused_z<T>(a, &pl->prm, 1, pl->amp, pl->phase);
+    if (d_step) a.d_time = pl->d_time;            // graph node: t = d_time[*d_step + j + 1]
+    else a.t_next = pl->h_time[(size_t)n + 1];
+    int rc = launch_march<T>(a, pl->variant, step_arg, st);
     if (rc) return rc;
-    WvBndArgs<T> b = make_bnd<T>(&pl->prm, pl->Nxy, pl->Nz, P(n + 1), P(n), (T*)pl->pmax, pl->amp, pl->phase);
-    b.d_step = d_step ? d_step : nullptr;
-    b.d_time = pl->d_time;
-    // eager steps pass the time directly; graph nodes read it from the table
-    if (!d_step) {
-        b.t = pl->h_time[(size_t)n + 1];
-    } else {
-        // the boundary kernel adds its node offset through d_time + offset
-        b.d_time = pl->d_time + step_arg;
-    }
-    b.fuse_source = 1;
-    return launch_bnd<T>(b, st);
+    const WvFaceArgs<T> f = make_faces<T>(&pl->prm, pl->Nxy, pl->Nz, P(n + 1), P(n), (T*)pl->pmax);
+    return launch_xyfaces<T>(f, st);
 }
 
 template <typename T>
@@ -898,7 +742,9 @@ static int plan_interior_only_t(bd_westervelt_plan* pl, hipStream_t st) {
     a.focus_idx = pl->focus_idx;
     a.g = pl->g;
     a.c = make_coef<T>(&pl->prm);
-    return launch_march<T>(a, n, st);
+    set_fused_z<T>(a, &pl->prm, 1, pl->amp, pl->phase);
+    a.t_next = pl->h_time[(size_t)n + 1];
+    return launch_march<T>(a, pl->variant, n, st);
 }
 
 extern "C" int bd_westervelt_plan_interior_only(bd_westervelt_plan* pl, void* stream) {
