@@ -60,6 +60,42 @@ class WesterveltParams(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
 
 
+class GilmoreParams(C.Structure):
+    """Mirror of bd_gilmore_params (G:13-33 plus integrator controls)."""
+    _fields_ = [(n, C.c_double) for n in (
+        "rho", "P0", "S2", "C_inf", "lambda_", "n", "B", "gamma1", "gamma3", "mu4", "G4_3")] + [
+        ("triple_fr", C.c_int32), ("tableau", C.c_int32),
+        ("minA", C.c_double), ("stepA", C.c_double), ("minR0", C.c_double), ("stepR0", C.c_double),
+        ("abs_eps", C.c_double), ("rel_eps", C.c_double), ("h0", C.c_double),
+        ("max_solve_steps", C.c_double), ("reserved", C.c_double * 7),
+    ]
+
+    def copy(self) -> "GilmoreParams":
+        o = GilmoreParams()
+        C.pointer(o)[0] = self
+        return o
+
+
+class GilmoreBatch(C.Structure):
+    """Mirror of bd_gilmore_batch."""
+    _fields_ = [("method", C.c_int32), ("precision", C.c_int32), ("n", C.c_int32),
+                ("out_every", C.c_int32), ("n_out", C.c_int32), ("block", C.c_int32),
+                ("chunk", C.c_int32), ("pad0", C.c_int32),
+                ("freq", C.c_double * 3), ("t_end", C.c_double), ("dt", C.c_double),
+                ("rtol", C.c_double), ("atol", C.c_double), ("split_ratio", C.c_double),
+                ("max_steps", C.c_longlong),
+                ("R0", C.c_void_p), ("amp", C.c_void_p), ("phase", C.c_void_p),
+                ("R_out", C.c_void_p), ("Rmax", C.c_void_p), ("t_collapse", C.c_void_p),
+                ("n_acc", C.c_void_p), ("n_rej", C.c_void_p), ("reserved", C.c_double * 4)]
+
+
+BD_GM_TAB_REFERENCE = 0
+BD_GM_TAB_DP1980 = 1
+BD_GM_RK4 = 0
+BD_GM_DP45 = 1
+BD_GM_FP64 = 0
+BD_GM_SPLIT = 1
+
 _VP = C.c_void_p
 _I = C.c_int
 _D = C.c_double
@@ -85,6 +121,10 @@ SIGNATURES: dict[str, tuple] = {
     "bd_westervelt_plan_destroy": (_I, [_VP]),
     "bd_westervelt_plan_interior_only": (_I, [_VP, _VP]),
     "bd_westervelt_plan_interior_cells": (C.c_longlong, [_VP]),
+    "bd_gilmore_reference_params": (None, [C.POINTER(GilmoreParams)]),
+    "bd_gilmore_threshold_sweep": (_I, [C.POINTER(GilmoreParams), _VP, _VP, _I, _I, _I, _VP]),
+    "bd_gilmore_workspace_bytes": (C.c_size_t, []),
+    "bd_gilmore_integrate": (_I, [C.POINTER(GilmoreParams), C.POINTER(GilmoreBatch), _VP, _VP]),
 }
 
 _lib = None

@@ -171,6 +171,77 @@ int bd_westervelt_plan_interior_only(bd_westervelt_plan* plan, void* stream);
 /* Number of cells the interior kernel updates per launch. */
 long long bd_westervelt_plan_interior_cells(const bd_westervelt_plan* plan);
 
+/* ======================================================================== */
+/*  Gilmore bubble dynamics in viscoelastic tissue                          */
+/* ======================================================================== */
+
+/* The reference's __device__ symbols (G:13-33) plus the integrator's
+ * controls.  bd_gilmore_reference_params() fills the liver values of G:440-460. */
+typedef struct bd_gilmore_params {
+    double rho, P0, S2, C_inf, lambda, n, B, gamma1, gamma3, mu4, G4_3;  /* G:13-23 */
+    int32_t triple_fr;                                                   /* G:26    */
+    int32_t tableau;           /* BD_GM_TAB_REFERENCE (G:218-227 as written, a32 = -9/40)
+                                  or BD_GM_TAB_DP1980 (published Dormand-Prince, a32 = +9/40) */
+    double minA, stepA, minR0, stepR0;                                   /* G:29-33 */
+    double abs_eps, rel_eps;   /* DP5(4) tolerances, G:214-215 (1e-9, 1e-9)       */
+    double h0;                 /* first step of every solve, G:355 / G:393 (1e-13) */
+    double max_solve_steps;    /* cap on attempted steps of one ODE solve (0 = 1e7); the
+                                  reference has none and can spin forever once h underflows */
+    double reserved[7];
+} bd_gilmore_params;
+
+#define BD_GM_TAB_REFERENCE 0
+#define BD_GM_TAB_DP1980    1
+
+void bd_gilmore_reference_params(bd_gilmore_params* p);
+
+/* Drop-in for DPRK_solver_threshold_fix_tissue<<<>>> (G:323-429): one result
+ * per (frequency combo, R0) pair, idx = i_combo*N_R0 + i_R0; the inertial-
+ * cavitation threshold amplitude found in 0.1 / 0.01 / 0.001 MPa stages, -1
+ * if none is set, 1e7 if no collapse up to 10 MPa.  Fr_combo: 3*N_fr_combo
+ * frequencies (device).  N_A is accepted and ignored, as in the reference. */
+int bd_gilmore_threshold_sweep(const bd_gilmore_params* prm, double* results_A,
+                               const double* Fr_combo, int N_fr_combo, int N_R0, int N_A,
+                               void* stream);
+
+#define BD_GM_RK4   0   /* classical fixed-step Runge-Kutta 4                       */
+#define BD_GM_DP45  1   /* adaptive Dormand-Prince 5(4), controller of G:212-318    */
+
+#define BD_GM_FP64   0  /* all arithmetic in FP64                                    */
+#define BD_GM_SPLIT  1  /* FP64 state/error norm; FP32 right-hand side while R/R0 >= split_ratio */
+
+/* One batch of independent bubbles driven by
+ * p(t) = sum_k amp[3i+k] sin(2 pi freq[k] t + phase[3i+k]).
+ * A bubble stops at t_end or at collapse (dR/dt <= -C_inf, the criterion of
+ * G:313).  All arrays are device memory. */
+typedef struct bd_gilmore_batch {
+    int32_t method, precision, n;
+    int32_t out_every;        /* RK4: record R every out_every steps (0 = no record)   */
+    int32_t n_out;            /* samples per bubble in R_out (NaN after collapse)      */
+    int32_t block;            /* workgroup size (0 = tuned default)                    */
+    int32_t chunk;            /* bubbles taken per queue grab (0 = tuned default)      */
+    int32_t pad0;
+    double freq[3];
+    double t_end, dt;         /* dt: RK4 step, DP45 first step                         */
+    double rtol, atol;        /* DP45 tolerances                                        */
+    double split_ratio;       /* BD_GM_SPLIT switches to FP64 RHS below R/R0 = ratio   */
+    long long max_steps;      /* per-bubble cap on attempted steps (0 = none)          */
+    const double* R0; const double* amp; const double* phase;
+    double* R_out;            /* n * n_out, nullable                                   */
+    double* Rmax;             /* n                                                       */
+    double* t_collapse;       /* n, -1 if no collapse before t_end                       */
+    long long* n_acc;         /* n, accepted steps                                      */
+    long long* n_rej;         /* n, rejected attempts (DP45)                            */
+    double reserved[4];
+} bd_gilmore_batch;
+
+/* Bytes of device workspace bd_gilmore_integrate needs (queue counter). */
+size_t bd_gilmore_workspace_bytes(void);
+/* Integrates a batch with a persistent work-queue kernel: each lane takes the
+ * next bubble from a device counter as soon as its previous one finished. */
+int bd_gilmore_integrate(const bd_gilmore_params* prm, const bd_gilmore_batch* batch,
+                         void* workspace, void* stream);
+
 /* Library metadata */
 const char* bd_version(void);
 int bd_device_count(void);

@@ -39,6 +39,20 @@ class Oracle:
             "orc_wv_state_elems": (C.c_size_t, [_I, _I]),
             "orc_wv_run": (_I, [PP, _I, _I, _I, _I, DP, DP, DP, DP, _I, _I, _I, DP, DP, _I]),
         }
+        from bubbledynamics_amd._ffi import GilmoreParams
+        GP = C.POINTER(GilmoreParams)
+        LP = C.POINTER(C.c_longlong)
+        sig.update({
+            "orc_gilmore_reference_params": (None, [GP]),
+            "orc_dp_tableau": (None, [_I, DP, DP, DP, DP]),
+            "orc_gilmore_freq_table": (_I, [_D, _D, _I, DP]),
+            "orc_gilmore_rhs": (None, [GP, _D, DP, DP, DP, _D, _D, DP, DP]),
+            "orc_gilmore_dprk_step": (_I, [GP, _D, DP, _D, _D, DP, DP, DP, C.POINTER(_I)]),
+            "orc_gilmore_threshold": (None, [GP, DP, DP, _I, _I, _I, _I, _I]),
+            "orc_gilmore_solve": (_I, [GP, _D, DP, _D, _D, _D, LP]),
+            "orc_gilmore_integrate": (None, [GP, _I, _I, DP, DP, DP, DP, _D, _D, _D, _D, _I, _I,
+                                             C.c_longlong, DP, DP, DP, LP, LP, _I, _I, _I]),
+        })
         for name, (res, args) in sig.items():
             f = getattr(self.lib, name)
             f.restype = res
@@ -71,6 +85,72 @@ class Oracle:
                fxyz=(-1, -1, -1), amp=None, phase=None, nthreads=0):
         return self.lib.orc_wv_run(C.byref(p), nxy, nz, step0, n_steps, _dp(state), _dp(pmax),
                                    _dp(energy), _dp(focus), *fxyz, _dp(amp), _dp(phase), nthreads)
+
+
+    # --- Gilmore --------------------------------------------------------
+    def gilmore_reference_params(self):
+        from bubbledynamics_amd._ffi import GilmoreParams
+        p = GilmoreParams()
+        self.lib.orc_gilmore_reference_params(C.byref(p))
+        return p
+
+    def dp_tableau(self, tab=1):
+        a = np.zeros(49); b = np.zeros(7); bb = np.zeros(7); c = np.zeros(7)
+        self.lib.orc_dp_tableau(tab, _dp(a), _dp(b), _dp(bb), _dp(c))
+        return a.reshape(7, 7), b, bb, c
+
+    def freq_table(self, min_fr=0.02e6, max_fr=5e6, n_fr=55):
+        n = self.lib.orc_gilmore_freq_table(min_fr, max_fr, n_fr, None)
+        out = np.zeros(3 * n)
+        self.lib.orc_gilmore_freq_table(min_fr, max_fr, n_fr, _dp(out))
+        return out.reshape(n, 3)
+
+    def gilmore_rhs(self, g, R0, y, t, freq, amp=None, phase=None, A_ref=0.0):
+        out = np.zeros(7)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        f = np.ascontiguousarray(freq, dtype=np.float64)
+        a = None if amp is None else np.ascontiguousarray(amp, dtype=np.float64)
+        ph = None if phase is None else np.ascontiguousarray(phase, dtype=np.float64)
+        self.lib.orc_gilmore_rhs(C.byref(g), R0, _dp(a), _dp(ph), _dp(f), A_ref, t, _dp(y), _dp(out))
+        return out
+
+    def dprk_step(self, g, R0, freq, A_ref, endT, t, y, h):
+        tt = np.array([t]); hh = np.array([h]); yy = np.array(y, dtype=np.float64)
+        f = np.ascontiguousarray(freq, dtype=np.float64)
+        thr = C.c_int(0)
+        rej = self.lib.orc_gilmore_dprk_step(C.byref(g), R0, _dp(f), A_ref, endT, _dp(tt), _dp(yy), _dp(hh), C.byref(thr))
+        return tt[0], yy, hh[0], bool(thr.value), rej
+
+    def gilmore_solve(self, g, R0, freq, A, endT, h0):
+        f = np.ascontiguousarray(freq, dtype=np.float64)
+        n = C.c_longlong(0)
+        c = self.lib.orc_gilmore_solve(C.byref(g), R0, _dp(f), A, endT, h0, C.byref(n))
+        return bool(c), n.value
+
+    def gilmore_threshold(self, g, fr_combo, n_r0, idx0, idx1, nthreads=0):
+        fr = np.ascontiguousarray(fr_combo, dtype=np.float64).ravel()
+        res = np.full(len(fr) // 3 * n_r0, np.nan)
+        self.lib.orc_gilmore_threshold(C.byref(g), _dp(res), _dp(fr), len(fr) // 3, n_r0, idx0, idx1, nthreads)
+        return res
+
+    def gilmore_integrate(self, g, method, R0, amp, phase, freq, t_end, dt, rtol=1e-8, atol=1e-12,
+                          out_every=0, n_out=0, max_steps=0, i0=0, i1=None, nthreads=0):
+        R0 = np.ascontiguousarray(R0, dtype=np.float64)
+        n = len(R0)
+        i1 = n if i1 is None else i1
+        amp = np.ascontiguousarray(amp, dtype=np.float64)
+        phase = np.ascontiguousarray(phase, dtype=np.float64)
+        freq = np.ascontiguousarray(freq, dtype=np.float64)
+        R_out = np.zeros(n * n_out) if n_out else None
+        Rmax = np.zeros(n); tc = np.zeros(n)
+        acc = np.zeros(n, dtype=np.int64); rej = np.zeros(n, dtype=np.int64)
+        LP = C.POINTER(C.c_longlong)
+        self.lib.orc_gilmore_integrate(C.byref(g), method, n, _dp(R0), _dp(amp), _dp(phase), _dp(freq),
+                                       t_end, dt, rtol, atol, out_every, n_out, max_steps,
+                                       _dp(R_out), _dp(Rmax), _dp(tc),
+                                       acc.ctypes.data_as(LP), rej.ctypes.data_as(LP), i0, i1, nthreads)
+        return dict(R_out=None if R_out is None else R_out.reshape(n, n_out), Rmax=Rmax,
+                    t_collapse=tc, n_acc=acc, n_rej=rej)
 
 
 _inst = None
