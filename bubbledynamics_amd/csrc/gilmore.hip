@@ -1,0 +1,626 @@
+// Gilmore bubble dynamics in viscoelastic tissue on gfx950.
+//
+// Reference: Gilmore_liver_3fr_combo.cu (G:line).
+//
+// Right-hand side (G:40-179), with the common subexpressions shared:
+//   p, dp/dt        one sincos per tone
+//   p_in, dp_in/dt  p_in = (P0 + 2s/R0) exp(3g log(R0/R)),  dp_in/dt = -3g p_in R'/R
+//   I^((n-1)/n) - 1 = expm1(((n-1)/n) log1p((u - v)/v)), u - v = p_in - 2s/R + tau - P0 - p
+//                   (B cancels exactly, so nothing cancels when I ~ 1)
+//   C = c_inf sqrt(I^((n-1)/n)),  I^(-1/n) = I^((n-1)/n) / I
+// Integrators: DP5(4) with the step controller of G:212-318 (the last stage
+// is reused as the next step's first, which the reference recomputes) and
+// classical RK4.  Kernels are persistent: every lane advances its own bubble
+// by one Runge-Kutta attempt per loop trip and takes the next bubble from a
+// device counter as soon as it finishes, so uneven step counts do not idle
+// lanes.  State, error norm and controller are FP64; with BD_GM_SPLIT the
+// stage right-hand sides run in FP32 while R/R0 >= split_ratio.
+#include "common.h"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+#include <vector>
+
+namespace bd {
+
+template <typename U>
+struct GmModel {
+    U rho, inv_rho, P0, S2, B, C_inf, inv_lambda, inv_3lambda, nm1_n, n_rho_nm1, gamma3, mu4, G4_3;
+};
+
+template <typename U>
+static GmModel<U> make_model(const bd_gilmore_params* g) {
+    GmModel<U> m;
+    m.rho = (U)g->rho;
+    m.inv_rho = (U)(1.0 / g->rho);
+    m.P0 = (U)g->P0;
+    m.S2 = (U)g->S2;
+    m.B = (U)g->B;
+    m.C_inf = (U)g->C_inf;
+    m.inv_lambda = (U)(1.0 / g->lambda);
+    m.inv_3lambda = (U)(1.0 / (3.0 * g->lambda));
+    m.nm1_n = (U)((g->n - 1.0) / g->n);
+    m.n_rho_nm1 = (U)(g->n / (g->rho * (g->n - 1.0)));
+    m.gamma3 = (U)g->gamma3;
+    m.mu4 = (U)g->mu4;
+    m.G4_3 = (U)g->G4_3;
+    return m;
+}
+
+// DP5(4) tableau (G:218-227): a, b (5th order, propagated), e = b - bb, c.
+struct GmTab {
+    double a[7][7], b[7], e[7], c[7];
+};
+
+static GmTab make_tab(int which) {
+    static const double A[7][7] = {
+        {0, 0, 0, 0, 0, 0, 0},
+        {0.2, 0, 0, 0, 0, 0, 0},
+        {3.0 / 40.0, -9.0 / 40.0, 0, 0, 0, 0, 0},
+        {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0, 0, 0, 0, 0},
+        {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0, 0, 0, 0},
+        {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0, 0, 0},
+        {35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0, 0}};
+    static const double Bv[7] = {35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0, 0.0};
+    static const double BB[7] = {5179.0 / 57600.0, 0.0, 7571.0 / 16695.0, 393.0 / 640.0, -92097.0 / 339200.0,
+                                 187.0 / 2100.0, 1.0 / 40.0};
+    static const double Cv[7] = {0.0, 0.2, 0.3, 0.8, 8.0 / 9.0, 1.0, 1.0};
+    GmTab t;
+    for (int i = 0; i < 7; ++i) {
+        for (int j = 0; j < 7; ++j) t.a[i][j] = A[i][j];
+        t.b[i] = Bv[i];
+        t.e[i] = Bv[i] - BB[i];
+        t.c[i] = Cv[i];
+    }
+    // the reference writes a32 = -9/40 (G:220); Dormand & Prince (1980) publish +9/40
+    if (which == BD_GM_TAB_DP1980) t.a[2][1] = 9.0 / 40.0;
+    return t;
+}
+
+struct GmConsts {
+    GmModel<double> m64;
+    GmModel<float> m32;
+    GmTab tab;
+    double C_inf, abs_eps, rel_eps, h0;
+    long long cap;
+};
+
+// p(t) = sum_k A[k] sin(w[k] t + ph[k])
+struct GmDrive {
+    double w[3], A[3], ph[3];
+};
+
+template <typename U>
+__device__ __forceinline__ void drive(const GmDrive& d, double t, U& p, U& pd) {
+    p = U(0);
+    pd = U(0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (d.A[k] == 0.0) continue;
+        double ph = d.w[k] * t + d.ph[k];
+        U s, c;
+        if constexpr (std::is_same<U, float>::value) {
+            // reduce in FP64, then an FP32 sincos on [-pi, pi]
+            ph = ph - 6.283185307179586 * rint(ph * 0.15915494309189535);
+            s = sinf((float)ph);
+            c = cosf((float)ph);
+        } else {
+            sincos(ph, &s, &c);
+        }
+        p += (U)d.A[k] * s;
+        pd += (U)(d.A[k] * d.w[k]) * c;
+    }
+}
+
+// State derivative (R', R'', tau', q') of G:161-179 in arithmetic U.
+template <typename U>
+__device__ __forceinline__ void gm_rhs(const GmModel<U>& m, const GmDrive& d, double t, U R0, U Pin0,
+                                       const U y[4], U f[4]) {
+    const U R = y[0], R1 = y[1], Tau = y[2], q = y[3];
+    U p, pd;
+    drive<U>(d, t, p, pd);
+    const U inv_R = U(1) / R;
+    const U x = R0 * inv_R;
+    const U pin = Pin0 * exp(m.gamma3 * log(x));                          // G:82-85
+    const U dpin = -m.gamma3 * pin * R1 * inv_R;                           // G:89-92
+    const U x3 = x * x * x;
+    const U el = -m.G4_3 * (U(1) - x3) - m.mu4 * R1 * inv_R;
+    const U tau1 = (el - Tau) * m.inv_lambda;                              // G:106-110
+    const U q1 = el * m.inv_3lambda - q * m.inv_lambda - R1 * Tau * inv_R;  // G:113-117
+    const U v = m.P0 + p + m.B;                                            // G:146
+    const U duv = pin - m.S2 * inv_R + Tau - m.P0 - p;                     // u - v
+    const U w = duv / v;                                                   // I - 1
+    const U L = log1p(w);
+    const U part2 = expm1(m.nm1_n * L);                                    // I^((n-1)/n) - 1
+    const U Ipow = U(1) + part2;
+    const U C = m.C_inf * sqrt(Ipow);                                      // G:96-103
+    const U part1 = v * m.n_rho_nm1;
+    const U H = part1 * part2;                                             // G:121-128
+    const U part1_dt = pd * m.n_rho_nm1;
+    const U u_dt = dpin + m.S2 * R1 * inv_R * inv_R + tau1;
+    const U I = U(1) + w;
+    const U I_dt = (u_dt - I * pd) / v;
+    const U part2_dt = m.nm1_n * (Ipow / I) * I_dt;
+    const U H1 = part1_dt * part2 + part1 * part2_dt;                      // G:132-157
+    const U inv_C = U(1) / C;
+    const U rc = R1 * inv_C;
+    const U a = U(1) - rc, b = U(1) + rc;
+    const U t1 = U(1.5) * R1 * R1 * (U(1) - rc * U(1.0 / 3.0));
+    const U t2 = b * (H + (U(3) * q - Tau) * m.inv_rho);
+    const U t3 = (H1 * a + (U(3) * q1 - tau1) * m.inv_rho) * R * inv_C;
+    f[0] = R1;
+    f[1] = (-t1 + t2 + t3) / (a * R);                                      // G:161-179
+    f[2] = tau1;
+    f[3] = q1;
+}
+
+// RHS on the FP64 state, evaluated in U.
+template <typename U>
+__device__ __forceinline__ void rhs_at(const GmConsts& K, const GmDrive& d, double t, double R0, double Pin0,
+                                       const double y[4], double f[4]) {
+    U yu[4], fu[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) yu[c] = (U)y[c];
+    if constexpr (std::is_same<U, float>::value)
+        gm_rhs<float>(K.m32, d, t, (float)R0, (float)Pin0, yu, fu);
+    else
+        gm_rhs<double>(K.m64, d, t, R0, Pin0, yu, fu);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) f[c] = (double)fu[c];
+}
+
+// One DP5(4) attempt from (t, y) with step h and first stage k1 (G:246-294).
+// Produces the 5th-order solution Y, the last stage k7 = f(t+h, .) and the
+// RMS error norm.  Sums follow the reference's accumulation order.
+template <typename U>
+__device__ __forceinline__ double dp_attempt(const GmConsts& K, const GmDrive& d, double R0, double Pin0,
+                                             double t, const double y[4], double h, const double k1[4],
+                                             double Y[4], double k7[4]) {
+    double k[7][4];
+    double E[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        k[0][c] = k1[c];
+        Y[c] = y[c] + K.tab.b[0] * k[0][c] * h;
+        E[c] = k[0][c] * h * K.tab.e[0];
+    }
+#pragma unroll
+    for (int i = 1; i < 7; ++i) {
+        double ys[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < i; ++j) acc = acc + K.tab.a[i][j] * k[j][c] * h;
+            ys[c] = y[c] + acc;
+        }
+        rhs_at<U>(K, d, t + h * K.tab.c[i], R0, Pin0, ys, k[i]);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            Y[c] = Y[c] + K.tab.b[i] * k[i][c] * h;
+            E[c] = E[c] + k[i][c] * h * K.tab.e[i];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) k7[c] = k[6][c];
+    double S = 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double sc = K.abs_eps + K.rel_eps * fmax(fabs(y[c]), fabs(Y[c]));
+        const double r = E[c] / sc;
+        S = S + r * r;
+    }
+    return sqrt(S / 4.0);                                                   // G:183-192
+}
+
+// Classical RK4 step (same operation order as the oracle).
+template <typename U>
+__device__ __forceinline__ void rk4_step(const GmConsts& K, const GmDrive& d, double R0, double Pin0, double t,
+                                         double h, double y[4]) {
+    double k1[4], k2[4], k3[4], k4[4], s[4];
+    rhs_at<U>(K, d, t, R0, Pin0, y, k1);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s[i] = y[i] + 0.5 * h * k1[i];
+    rhs_at<U>(K, d, t + 0.5 * h, R0, Pin0, s, k2);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s[i] = y[i] + 0.5 * h * k2[i];
+    rhs_at<U>(K, d, t + 0.5 * h, R0, Pin0, s, k3);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s[i] = y[i] + h * k3[i];
+    rhs_at<U>(K, d, t + h, R0, Pin0, s, k4);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) y[i] = y[i] + h / 6.0 * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
+}
+
+// G:296: max(0.1, 0.9 err^-0.2)
+__device__ __forceinline__ double step_factor(double err) { return fmax(0.1, 0.9 * pow(err, -0.2)); }
+
+// ---------------------------------------------------------------------------
+// batch integrator (persistent, per-lane refill)
+// ---------------------------------------------------------------------------
+struct GmBatchArgs {
+    GmConsts K;
+    int method, precision, n, out_every, n_out, chunk;
+    double w[3];
+    double t_end, dt, split_ratio;
+    long long max_steps;
+    const double* R0; const double* amp; const double* phase;
+    double* R_out; double* Rmax; double* t_collapse;
+    long long* n_acc; long long* n_rej;
+    int* counter;
+};
+
+template <int METHOD, int PREC>
+__global__ void __launch_bounds__(256) gm_batch_kernel(GmBatchArgs a) {
+    const GmConsts& K = a.K;
+    int next = 0, end = 0;          // this lane's current chunk of bubble indices
+    bool have = false;
+    int i = 0;
+    GmDrive d;
+    double R0 = 0, Pin0 = 0, t = 0, h = 0, rmax = 0, tc = -1, fac_max = 5.0;
+    double y[4] = {0, 0, 0, 0}, k1[4] = {0, 0, 0, 0};
+    bool k1_ok = false, at_start = true;
+    long long acc = 0, rej = 0, s_rk = 0, nsteps_rk = 0;
+    int j_out = 0;
+    while (true) {
+        if (!have) {
+            if (next >= end) {
+                next = atomicAdd(a.counter, a.chunk);
+                end = min(next + a.chunk, a.n);
+                if (next >= a.n) break;
+            }
+            i = next++;
+            have = true;
+            R0 = a.R0[i];
+            Pin0 = K.m64.P0 + K.m64.S2 / R0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) { d.w[k] = a.w[k]; d.A[k] = a.amp[3 * i + k]; d.ph[k] = a.phase[3 * i + k]; }
+            y[0] = R0; y[1] = y[2] = y[3] = 0.0;
+            t = 0.0; h = a.dt; rmax = R0; tc = -1.0; fac_max = 5.0;
+            k1_ok = false; at_start = true;
+            acc = rej = 0; s_rk = 0;
+            nsteps_rk = llrint(a.t_end / a.dt);
+            j_out = 0;
+            if (a.R_out && a.n_out > 0) a.R_out[(size_t)i * a.n_out + j_out++] = R0;
+        }
+        bool done = false;
+        const bool use32 = PREC == BD_GM_SPLIT && y[0] >= a.split_ratio * R0;
+        if constexpr (METHOD == BD_GM_RK4) {
+            if (PREC == BD_GM_SPLIT && use32) rk4_step<float>(K, d, R0, Pin0, s_rk * a.dt, a.dt, y);
+            else rk4_step<double>(K, d, R0, Pin0, s_rk * a.dt, a.dt, y);
+            ++s_rk; ++acc;
+            t = s_rk * a.dt;
+            if (y[0] > rmax) rmax = y[0];
+            if (a.R_out && a.out_every > 0 && s_rk % a.out_every == 0 && j_out < a.n_out)
+                a.R_out[(size_t)i * a.n_out + j_out++] = y[0];
+            if (y[1] <= -K.C_inf) { tc = t; done = true; }
+            if (s_rk >= nsteps_rk || (a.max_steps > 0 && acc >= a.max_steps)) done = true;
+        } else {
+            if (!k1_ok) {
+                if (PREC == BD_GM_SPLIT && use32) rhs_at<float>(K, d, t, R0, Pin0, y, k1);
+                else rhs_at<double>(K, d, t, R0, Pin0, y, k1);
+                k1_ok = true;
+            }
+            if (at_start) {
+                if (t + h > a.t_end) h = a.t_end - t;
+                fac_max = 5.0;
+                at_start = false;
+            }
+            double Y[4], k7[4];
+            double err;
+            if (PREC == BD_GM_SPLIT && use32) err = dp_attempt<float>(K, d, R0, Pin0, t, y, h, k1, Y, k7);
+            else err = dp_attempt<double>(K, d, R0, Pin0, t, y, h, k1, Y, k7);
+            const double mr = step_factor(err);
+            bool accept = !(err > 1.0);
+            if (!accept) {
+                fac_max = 0.9;
+                h = h * fmin(fac_max, mr);
+                ++rej;
+                // h underflow or non-finite error: take the attempt as it is
+                if (!(h > 0.0) || !isfinite(err)) accept = true;
+            }
+            if (accept) {
+                t = t + h;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) { y[c] = Y[c]; k1[c] = k7[c]; }
+                h = h * fmin(fac_max, mr);
+                at_start = true;
+                ++acc;
+                if (y[0] > rmax) rmax = y[0];
+                if (y[1] <= -K.C_inf) { tc = t; done = true; }
+                if (!(t < a.t_end) || !(h > 0.0) || !isfinite(y[0])) done = true;
+                if (a.max_steps > 0 && acc + rej >= a.max_steps) done = true;
+            }
+        }
+        if (done) {
+            if (a.R_out)
+                for (; j_out < a.n_out; ++j_out) a.R_out[(size_t)i * a.n_out + j_out] = NAN;
+            a.Rmax[i] = rmax;
+            a.t_collapse[i] = tc;
+            if (a.n_acc) a.n_acc[i] = acc;
+            if (a.n_rej) a.n_rej[i] = rej;
+            have = false;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// drop-in threshold sweep (G:323-429), persistent with per-lane refill
+// ---------------------------------------------------------------------------
+struct GmSweepArgs {
+    GmConsts K;
+    int triple, N_fr_combo, N_R0;
+    double minR0, stepR0;
+    double* results_A;
+    const double* Fr_combo;
+    int* counter;
+};
+
+// Reference drive (G:40-57) as amplitudes per tone; the dual mode uses fr1 and fr3.
+__device__ __forceinline__ void ref_drive(GmDrive& d, bool triple, const double fr[3], double A) {
+    for (int k = 0; k < 3; ++k) { d.w[k] = 2.0 * M_PI * fr[k]; d.A[k] = 0.0; d.ph[k] = 0.0; }
+    if (!triple || (fr[0] == fr[1] && fr[0] == fr[2])) {
+        d.A[0] = A;
+    } else if (fr[0] == fr[1] || fr[1] == fr[2]) {
+        d.A[0] = A / sqrt(2.0);
+        d.A[2] = A / sqrt(2.0);
+    } else {
+        d.A[0] = d.A[1] = d.A[2] = A / sqrt(3.0);
+    }
+}
+
+__global__ void __launch_bounds__(256) gm_sweep_kernel(GmSweepArgs a) {
+    const GmConsts& K = a.K;
+    const int N_total = a.N_fr_combo * a.N_R0;
+    const double A_MAX = 1.0e7;
+    bool have = false;
+    int idx = 0;
+    double fr[3] = {0, 0, 0};
+    double R0 = 0, Pin0 = 0, endT = 0, t = 0, h = 0, fac_max = 5.0;
+    double y[4] = {0, 0, 0, 0}, k1[4] = {0, 0, 0, 0};
+    bool k1_ok = false, at_start = true;
+    int stage = 0;          // 1, 2, 3: search stage of G:371-426
+    double A5 = 0, A4 = 0, A3 = 0, result = -1.0;
+    bool thr5 = false, thr4 = true, thr3 = true;
+    long long steps = 0;
+    GmDrive d;
+
+    auto start_solve = [&](double A, bool reset_h) {
+        if (reset_h) h = K.h0;
+        t = 0.0; y[0] = R0; y[1] = y[2] = y[3] = 0.0;
+        ref_drive(d, a.triple != 0, fr, A);
+        k1_ok = false; at_start = true; steps = 0;
+    };
+
+    while (true) {
+        if (!have) {
+            idx = atomicAdd(a.counter, 1);
+            if (idx >= N_total) break;
+            have = true;
+            const int ic = idx / a.N_R0, ir = idx % a.N_R0;
+            R0 = a.minR0 + ir * a.stepR0;
+            Pin0 = K.m64.P0 + K.m64.S2 / R0;
+            fr[0] = a.Fr_combo[3 * ic]; fr[1] = a.Fr_combo[3 * ic + 1]; fr[2] = a.Fr_combo[3 * ic + 2];
+            if (a.triple) {  // G:342-353
+                if (fr[0] <= fr[1] && fr[0] <= fr[2]) endT = 5.0 / fr[0];
+                if (fr[1] <= fr[0] && fr[1] <= fr[2]) endT = 5.0 / fr[1];
+                if (fr[2] <= fr[0] && fr[2] <= fr[1]) endT = 5.0 / fr[2];
+            } else {
+                endT = 5.0 / fr[0];
+            }
+            h = K.h0;
+            result = -1.0;
+            thr5 = false; thr4 = true; thr3 = true;
+            A5 = A4 = A3 = 0.0;
+            stage = 1;
+            A5 = A5 + 1.0e5;
+            start_solve(A5, false);
+        }
+        // one DP5(4) attempt of the current solve
+        if (!k1_ok) { rhs_at<double>(K, d, t, R0, Pin0, y, k1); k1_ok = true; }
+        if (at_start) {
+            if (t + h > endT) h = endT - t;
+            fac_max = 5.0;
+            at_start = false;
+        }
+        double Y[4], k7[4];
+        const double err = dp_attempt<double>(K, d, R0, Pin0, t, y, h, k1, Y, k7);
+        const double mr = step_factor(err);
+        bool accept = !(err > 1.0);
+        if (!accept) {
+            fac_max = 0.9;
+            h = h * fmin(fac_max, mr);
+            if (!(h > 0.0) || !isfinite(err)) accept = true;
+        }
+        if (!accept) continue;
+        t = t + h;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) { y[c] = Y[c]; k1[c] = k7[c]; }
+        h = h * fmin(fac_max, mr);
+        at_start = true;
+        ++steps;
+        const bool thr = Y[1] <= -K.C_inf;
+        if (t < endT && !thr && steps < K.cap) continue;
+        // the solve is over: advance the search (G:372-426)
+        if (stage == 1) {
+            thr5 = thr;
+            if (!thr5 && A5 < A_MAX) { A5 = A5 + 1.0e5; start_solve(A5, false); continue; }
+            if (!thr5 && A5 == A_MAX) result = A5;
+            A4 = A5;
+            if (thr5 && thr4) { stage = 2; A4 = A4 - 1.0e4; thr4 = false; start_solve(A4, true); continue; }
+            stage = 3;
+        } else if (stage == 2) {
+            thr4 = thr;
+            if (thr4) { A4 = A4 - 1.0e4; thr4 = false; start_solve(A4, true); continue; }
+            A4 = A4 + 1.0e4;
+            A3 = A4;
+            if (thr5 && !thr4 && thr3) { stage = 3; A3 = A3 - 1.0e3; thr3 = false; start_solve(A3, true); continue; }
+        } else {
+            thr3 = thr;
+            if (!thr3) {
+                result = A3 + 1.0e3;
+            } else {
+                A3 = A3 - 1.0e3; thr3 = false; start_solve(A3, true); continue;
+            }
+        }
+        a.results_A[idx] = result;
+        have = false;
+    }
+}
+
+__global__ void zero_int_kernel(int* p) { if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0; }
+
+static GmConsts make_consts(const bd_gilmore_params* g) {
+    GmConsts K;
+    K.m64 = make_model<double>(g);
+    K.m32 = make_model<float>(g);
+    K.tab = make_tab(g->tableau);
+    K.C_inf = g->C_inf;
+    K.abs_eps = g->abs_eps;
+    K.rel_eps = g->rel_eps;
+    K.h0 = g->h0;
+    K.cap = g->max_solve_steps > 0 ? (long long)g->max_solve_steps : 10000000LL;
+    return K;
+}
+
+static int persistent_blocks(int block) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t pr;
+        if (hipGetDeviceProperties(&pr, dev) == hipSuccess && pr.multiProcessorCount > 0) cus = pr.multiProcessorCount;
+    }
+    // enough waves per SIMD to hide FP64 latency; bounded by the work
+    return cus * std::max(1, 2048 / block);
+}
+
+// one counter per device for the drop-in sweep (its signature has no workspace)
+static int* sweep_counter() {
+    static int* ctr[64] = {nullptr};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    if (!ctr[dev] && hipMalloc(&ctr[dev], sizeof(int)) != hipSuccess) ctr[dev] = nullptr;
+    return ctr[dev];
+}
+
+}  // namespace bd
+
+using namespace bd;
+
+extern "C" void bd_gilmore_reference_params(bd_gilmore_params* p) {  // G:440-460
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->rho = 1100.0;
+    p->P0 = 1.013 * std::pow(10.0, 5);
+    p->S2 = 2.0 * 0.056;
+    p->mu4 = 4.0 * 0.009;
+    p->G4_3 = 0.04 * std::pow(10.0, 6) * 4.0 / 3.0;
+    p->C_inf = 1549.0;
+    p->lambda = 3.0 * std::pow(10.0, -9);
+    p->n = 7.0;
+    p->B = (p->C_inf * p->C_inf * p->rho / p->n) - p->P0;
+    p->gamma1 = 1.4;
+    p->gamma3 = 3.0 * p->gamma1;
+    p->triple_fr = 1;
+    p->tableau = BD_GM_TAB_REFERENCE;
+    p->minA = std::pow(10.0, 4);
+    p->stepA = 1000.0;
+    p->stepR0 = std::pow(10.0, -7);
+    p->minR0 = std::pow(10.0, -7);
+    p->abs_eps = std::pow(10.0, -9);
+    p->rel_eps = std::pow(10.0, -9);
+    p->h0 = std::pow(10.0, -13);
+    p->max_solve_steps = 1e7;
+}
+
+static int check_gilmore(const bd_gilmore_params* g) {
+    if (!g) return BD_ERR_INVALID;
+    if (!(g->rho > 0) || !(g->lambda > 0) || !(g->n > 1) || !(g->C_inf > 0) || !(g->h0 > 0))
+        return BD_ERR_INVALID;
+    if (g->tableau != BD_GM_TAB_REFERENCE && g->tableau != BD_GM_TAB_DP1980) return BD_ERR_INVALID;
+    return BD_OK;
+}
+
+extern "C" int bd_gilmore_threshold_sweep(const bd_gilmore_params* prm, double* results_A,
+                                          const double* Fr_combo, int N_fr_combo, int N_R0, int N_A,
+                                          void* stream) {
+    (void)N_A;
+    int rc = check_gilmore(prm);
+    if (rc) return rc;
+    if (!results_A || !Fr_combo || N_fr_combo <= 0 || N_R0 <= 0) return BD_ERR_INVALID;
+    if ((long long)N_fr_combo * N_R0 > 0x7fffffffLL) return BD_ERR_INVALID;
+    int* ctr = sweep_counter();
+    if (!ctr) return BD_ERR_ALLOC;
+    hipStream_t st = as_stream(stream);
+    GmSweepArgs a;
+    a.K = make_consts(prm);
+    a.triple = prm->triple_fr;
+    a.N_fr_combo = N_fr_combo;
+    a.N_R0 = N_R0;
+    a.minR0 = prm->minR0;
+    a.stepR0 = prm->stepR0;
+    a.results_A = results_A;
+    a.Fr_combo = Fr_combo;
+    a.counter = ctr;
+    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, st, ctr);
+    BD_LAUNCH_CHECK();
+    const long long total = (long long)N_fr_combo * N_R0;
+    const int block = 64;
+    int blocks = persistent_blocks(block);
+    blocks = (int)std::min<long long>(blocks, (total + block - 1) / block);
+    hipLaunchKernelGGL(gm_sweep_kernel, dim3(blocks), dim3(block), 0, st, a);
+    BD_LAUNCH_CHECK();
+    return BD_OK;
+}
+
+extern "C" size_t bd_gilmore_workspace_bytes(void) { return 256; }
+
+extern "C" int bd_gilmore_integrate(const bd_gilmore_params* prm, const bd_gilmore_batch* b, void* workspace,
+                                    void* stream) {
+    int rc = check_gilmore(prm);
+    if (rc) return rc;
+    if (!b || !workspace || b->n <= 0 || !b->R0 || !b->amp || !b->phase || !b->Rmax || !b->t_collapse)
+        return BD_ERR_INVALID;
+    if (b->method != BD_GM_RK4 && b->method != BD_GM_DP45) return BD_ERR_INVALID;
+    if (b->precision != BD_GM_FP64 && b->precision != BD_GM_SPLIT) return BD_ERR_INVALID;
+    if (!(b->dt > 0) || !(b->t_end > 0)) return BD_ERR_INVALID;
+    if (b->R_out && b->n_out <= 0) return BD_ERR_INVALID;
+    if (b->method == BD_GM_DP45 && (!(b->rtol > 0) || !(b->atol >= 0))) return BD_ERR_INVALID;
+    hipStream_t st = as_stream(stream);
+    GmBatchArgs a;
+    a.K = make_consts(prm);
+    if (b->method == BD_GM_DP45) { a.K.abs_eps = b->atol; a.K.rel_eps = b->rtol; }
+    a.method = b->method;
+    a.precision = b->precision;
+    a.n = b->n;
+    a.out_every = b->out_every;
+    a.n_out = b->R_out ? b->n_out : 0;
+    a.chunk = b->chunk > 0 ? b->chunk : 1;
+    for (int k = 0; k < 3; ++k) a.w[k] = 2.0 * M_PI * b->freq[k];
+    a.t_end = b->t_end;
+    a.dt = b->dt;
+    a.split_ratio = b->split_ratio;
+    a.max_steps = b->max_steps > 0 ? b->max_steps : 10000000LL;   // every lane terminates
+    a.R0 = b->R0; a.amp = b->amp; a.phase = b->phase;
+    a.R_out = b->R_out; a.Rmax = b->Rmax; a.t_collapse = b->t_collapse;
+    a.n_acc = b->n_acc; a.n_rej = b->n_rej;
+    a.counter = (int*)workspace;
+    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, st, a.counter);
+    BD_LAUNCH_CHECK();
+    const int block = b->block > 0 ? b->block : 256;
+    if (block % kWave != 0 || block > 256) return BD_ERR_INVALID;
+    int blocks = persistent_blocks(block);
+    const long long need = ((long long)b->n + (long long)a.chunk - 1) / a.chunk;
+    blocks = (int)std::min<long long>(blocks, (need + block - 1) / block);
+    const dim3 grid(std::max(1, blocks)), blk(block);
+    if (b->method == BD_GM_RK4 && b->precision == BD_GM_FP64)
+        hipLaunchKernelGGL((gm_batch_kernel<BD_GM_RK4, BD_GM_FP64>), grid, blk, 0, st, a);
+    else if (b->method == BD_GM_RK4)
+        hipLaunchKernelGGL((gm_batch_kernel<BD_GM_RK4, BD_GM_SPLIT>), grid, blk, 0, st, a);
+    else if (b->precision == BD_GM_FP64)
+        hipLaunchKernelGGL((gm_batch_kernel<BD_GM_DP45, BD_GM_FP64>), grid, blk, 0, st, a);
+    else
+        hipLaunchKernelGGL((gm_batch_kernel<BD_GM_DP45, BD_GM_SPLIT>), grid, blk, 0, st, a);
+    BD_LAUNCH_CHECK();
+    return BD_OK;
+}

@@ -1,0 +1,160 @@
+"""Host-side mirror of the reference's Gilmore driver (G:433-631) and the
+batched bubble workloads, over the C ABI.
+
+Device memory comes from torch tensors; the integration itself is the HIP
+library's persistent work-queue kernel.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from . import _ffi
+from ._ffi import (BD_GM_DP45, BD_GM_FP64, BD_GM_RK4, BD_GM_SPLIT, BD_GM_TAB_DP1980,
+                   BD_GM_TAB_REFERENCE, GilmoreBatch, GilmoreParams, call)
+
+
+def reference_params() -> GilmoreParams:
+    """Liver constants of G:440-460, reference tableau and controller."""
+    p = GilmoreParams()
+    _ffi.load().bd_gilmore_reference_params(C.byref(p))
+    return p
+
+
+def workload_params(published_tableau: bool = True) -> GilmoreParams:
+    """Medium of the batched workloads: rho=1060, c=1590, sigma=0.056,
+    mu=0.005, B=3.3e8, n=7, kappa=1.4; P0, G and lambda from the reference
+    (G:441, G:444, G:446)."""
+    p = reference_params()
+    p.rho = 1060.0
+    p.C_inf = 1590.0
+    p.S2 = 2 * 0.056
+    p.mu4 = 4 * 0.005
+    p.B = 3.3e8
+    p.n = 7.0
+    p.gamma1 = 1.4
+    p.gamma3 = 4.2
+    p.tableau = BD_GM_TAB_DP1980 if published_tableau else BD_GM_TAB_REFERENCE
+    return p
+
+
+def freq_table(min_fr=0.02e6, max_fr=5e6, n_fr=55) -> np.ndarray:
+    """Frequency triplets of G:465-469, G:498-520 (27,775 combos for 55 tones)."""
+    step_log = (math.log10(max_fr) - math.log10(min_fr)) / (n_fr - 1.0)
+    f = [10.0 ** (math.log10(min_fr) + i * step_log) for i in range(n_fr)]
+    out = []
+    for i in range(n_fr):
+        first = True
+        for ii in range(i, n_fr):
+            iii = ii if first else ii + 1
+            for k in range(iii, n_fr):
+                out.append((f[i], f[ii], f[k]))
+            first = False
+    return np.array(out)
+
+
+def _vp(t):
+    return C.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def threshold_sweep(params: GilmoreParams, fr_combo: np.ndarray, n_r0: int) -> np.ndarray:
+    """Drop-in run of DPRK_solver_threshold_fix_tissue (G:323-429)."""
+    fr = torch.as_tensor(np.ascontiguousarray(fr_combo, dtype=np.float64).ravel(), device="cuda")
+    n_combo = fr.numel() // 3
+    res = torch.empty(n_combo * n_r0, dtype=torch.float64, device="cuda")
+    call("bd_gilmore_threshold_sweep", C.byref(params), _vp(res), _vp(fr), n_combo, n_r0, 0, _stream())
+    torch.cuda.synchronize()
+    return res.cpu().numpy()
+
+
+class BubbleBatch:
+    """A batch of bubbles with per-bubble R0 and three-tone drive amplitudes and
+    phases on shared frequencies; outputs live on the GPU."""
+
+    def __init__(self, R0, amp, phase, freq):
+        self.n = len(R0)
+        self.R0 = torch.as_tensor(np.ascontiguousarray(R0, dtype=np.float64), device="cuda")
+        self.amp = torch.as_tensor(np.ascontiguousarray(amp, dtype=np.float64).reshape(-1), device="cuda")
+        self.phase = torch.as_tensor(np.ascontiguousarray(phase, dtype=np.float64).reshape(-1), device="cuda")
+        self.freq = tuple(float(f) for f in freq)
+        self.workspace = torch.zeros(int(_ffi.load().bd_gilmore_workspace_bytes()), dtype=torch.uint8,
+                                     device="cuda")
+
+    def run(self, params: GilmoreParams, method: int, t_end: float, dt: float, *, precision=BD_GM_FP64,
+            rtol=1e-8, atol=1e-12, split_ratio=0.2, out_every=0, n_out=0, max_steps=0, block=0,
+            chunk=0) -> dict:
+        n = self.n
+        dev = "cuda"
+        out = {
+            "Rmax": torch.empty(n, dtype=torch.float64, device=dev),
+            "t_collapse": torch.empty(n, dtype=torch.float64, device=dev),
+            "n_acc": torch.empty(n, dtype=torch.int64, device=dev),
+            "n_rej": torch.empty(n, dtype=torch.int64, device=dev),
+            "R_out": torch.empty((n, n_out), dtype=torch.float64, device=dev) if n_out else None,
+        }
+        b = GilmoreBatch()
+        b.method, b.precision, b.n = method, precision, n
+        b.out_every, b.n_out = out_every, n_out
+        b.block, b.chunk = block, chunk
+        for k in range(3):
+            b.freq[k] = self.freq[k]
+        b.t_end, b.dt, b.rtol, b.atol, b.split_ratio = t_end, dt, rtol, atol, split_ratio
+        b.max_steps = max_steps
+        b.R0, b.amp, b.phase = self.R0.data_ptr(), self.amp.data_ptr(), self.phase.data_ptr()
+        b.R_out = out["R_out"].data_ptr() if n_out else None
+        b.Rmax, b.t_collapse = out["Rmax"].data_ptr(), out["t_collapse"].data_ptr()
+        b.n_acc, b.n_rej = out["n_acc"].data_ptr(), out["n_rej"].data_ptr()
+        call("bd_gilmore_integrate", C.byref(params), C.byref(b), _vp(self.workspace), _stream())
+        return out
+
+
+def seeded_uniforms(seed: int, n: int) -> np.ndarray:
+    """SplitMix64 -> 53-bit uniforms in [0, 1), the seeded inputs of the workloads."""
+    out = np.empty(n)
+    s = seed & 0xFFFFFFFFFFFFFFFF
+    M = 0xFFFFFFFFFFFFFFFF
+    for i in range(n):
+        s = (s + 0x9E3779B97F4A7C15) & M
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z = z ^ (z >> 31)
+        out[i] = (z >> 11) * (1.0 / 9007199254740992.0)
+    return out
+
+
+def config0_inputs(n: int = 1024, seed: int = 0):
+    """1,024 bubbles: R0 ~ U(1,5) um, A_k ~ U(0.5,3) MPa, phi_k ~ U(0,2pi); f = (1,2,3) MHz.
+    Draw order per bubble: R0, A1..A3, phi1..phi3."""
+    u = seeded_uniforms(seed, 7 * n).reshape(n, 7)
+    R0 = 1e-6 + 4e-6 * u[:, 0]
+    amp = 0.5e6 + 2.5e6 * u[:, 1:4]
+    ph = 2 * math.pi * u[:, 4:7]
+    return R0, amp, ph, (1e6, 2e6, 3e6)
+
+
+def config3_inputs(n: int = 1_000_000, seed: int = 3):
+    """1e6 bubbles: R0 log-uniform 0.5-20 um, A_k ~ U(0.5,8) MPa, phi_k ~ U(0,2pi)."""
+    u = _splitmix_vec(seed, 7 * n).reshape(n, 7)
+    R0 = np.exp(math.log(0.5e-6) + u[:, 0] * math.log(40.0))
+    amp = 0.5e6 + 7.5e6 * u[:, 1:4]
+    ph = 2 * math.pi * u[:, 4:7]
+    return R0, amp, ph, (1e6, 2e6, 3e6)
+
+
+def _splitmix_vec(seed: int, n: int) -> np.ndarray:
+    """Vectorised SplitMix64 (identical stream to seeded_uniforms)."""
+    with np.errstate(over="ignore"):
+        idx = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)

@@ -225,7 +225,7 @@ typedef struct bd_gilmore_batch {
     double t_end, dt;         /* dt: RK4 step, DP45 first step                         */
     double rtol, atol;        /* DP45 tolerances                                        */
     double split_ratio;       /* BD_GM_SPLIT switches to FP64 RHS below R/R0 = ratio   */
-    long long max_steps;      /* per-bubble cap on attempted steps (0 = none)          */
+    long long max_steps;      /* per-bubble cap on attempted steps (0 = 1e7)           */
     const double* R0; const double* amp; const double* phase;
     double* R_out;            /* n * n_out, nullable                                   */
     double* Rmax;             /* n                                                       */

@@ -454,6 +454,7 @@ void orc_gilmore_integrate(const bd_gilmore_params* g, int method, int n, const 
     (void)nthreads;
 #endif
     (void)n;
+    if (max_steps <= 0) max_steps = 10000000LL;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int i = i0; i < i1; ++i) {
         Model m{g, {}, R0[i]};

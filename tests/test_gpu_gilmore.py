@@ -1,0 +1,98 @@
+"""GPU parity of the Gilmore kernels against the FP64 oracle (through the C ABI)."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from bubbledynamics_amd import gilmore as gm  # noqa: E402
+from bubbledynamics_amd._ffi import BD_GM_DP45, BD_GM_FP64, BD_GM_RK4, BD_GM_SPLIT  # noqa: E402
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def test_threshold_sweep_matches_oracle(orc):
+    # drop-in DPRK_solver_threshold_fix_tissue on 2 combos x 4 radii
+    p = gm.reference_params()
+    p.minR0, p.stepR0 = 1e-6, 1e-6
+    fr = np.array([[1e6, 1e6, 1e6], [1e6, 2e6, 3e6]])
+    n_r0 = 4
+    got = gm.threshold_sweep(p, fr, n_r0)
+    ref = orc.gilmore_threshold(p, fr, n_r0, 0, 2 * n_r0, nthreads=8)
+    print("GPU", got, "\nCPU", ref)
+    assert np.all(np.abs(got - ref) <= 1e3)
+    assert np.sum(got == ref) >= 6
+
+
+def _batch(n, seed, t_end_us):
+    R0, amp, ph, fr = gm.config0_inputs(n, seed)
+    return R0, amp, ph, fr, t_end_us * 1e-6
+
+
+def test_rk4_batch_matches_oracle(orc):
+    p = gm.workload_params()
+    R0, amp, ph, fr, T = _batch(64, 0, 2.0)
+    b = gm.BubbleBatch(R0, amp, ph, fr)
+    out = b.run(p, BD_GM_RK4, T, 1e-10, out_every=10, n_out=2001)
+    torch.cuda.synchronize()
+    ref = orc.gilmore_integrate(p, BD_GM_RK4, R0, amp, ph, fr, T, 1e-10, out_every=10, n_out=2001, nthreads=8)
+    np.testing.assert_allclose(host(out["Rmax"]), ref["Rmax"], rtol=1e-9)
+    np.testing.assert_array_equal(host(out["n_acc"]), ref["n_acc"])
+    tc, tcr = host(out["t_collapse"]), ref["t_collapse"]
+    np.testing.assert_allclose(tc, tcr, rtol=1e-12)
+    Ro, Rr = host(out["R_out"]), ref["R_out"]
+    np.testing.assert_array_equal(np.isnan(Ro), np.isnan(Rr))
+    fin = ~np.isnan(Rr)
+    np.testing.assert_allclose(Ro[fin], Rr[fin], rtol=1e-8)
+    assert (tcr > 0).any() and (tcr < 0).any()       # both collapsing and surviving bubbles
+
+
+@pytest.mark.parametrize("tab", [0, 1])
+def test_dp45_batch_matches_oracle(orc, tab):
+    p = gm.workload_params()
+    p.tableau = tab
+    R0, amp, ph, fr = gm.config3_inputs(48, 3)
+    T = 3e-6
+    b = gm.BubbleBatch(R0, amp, ph, fr)
+    out = b.run(p, BD_GM_DP45, T, 1e-13, rtol=1e-8, atol=1e-12)
+    torch.cuda.synchronize()
+    ref = orc.gilmore_integrate(p, BD_GM_DP45, R0, amp, ph, fr, T, 1e-13, rtol=1e-8, atol=1e-12, nthreads=8)
+    rel = np.abs(host(out["Rmax"]) / ref["Rmax"] - 1)
+    print("DP45 Rmax rel err max", rel.max(), "median", np.median(rel))
+    assert np.median(rel) <= 1e-9
+    assert rel.max() <= 1e-6
+    acc, acc_r = host(out["n_acc"]), ref["n_acc"]
+    assert np.all(np.abs(acc - acc_r) <= np.maximum(3, 0.01 * acc_r))
+    np.testing.assert_array_equal(host(out["t_collapse"]) > 0, ref["t_collapse"] > 0)
+
+
+def test_split_precision_rmax_error():
+    # FP32 right-hand sides while R/R0 >= 0.2, FP64 below; Rmax against all-FP64
+    p = gm.workload_params()
+    R0, amp, ph, fr = gm.config3_inputs(2048, 3)
+    T = 20e-6
+    b = gm.BubbleBatch(R0, amp, ph, fr)
+    o64 = b.run(p, BD_GM_DP45, T, 1e-13, rtol=1e-8, atol=1e-12)
+    o32 = b.run(p, BD_GM_DP45, T, 1e-13, precision=BD_GM_SPLIT, rtol=1e-8, atol=1e-12)
+    torch.cuda.synchronize()
+    rel = np.abs(host(o32["Rmax"]) / host(o64["Rmax"]) - 1)
+    print(f"split Rmax rel err: max {rel.max():.3e} median {np.median(rel):.3e} "
+          f"p99 {np.quantile(rel, 0.99):.3e}; steps fp64 {host(o64['n_acc']).sum()} split {host(o32['n_acc']).sum()}")
+    assert np.median(rel) <= 1e-6
+
+
+def test_queue_scheduling_does_not_change_results():
+    p = gm.workload_params()
+    R0, amp, ph, fr = gm.config3_inputs(512, 3)
+    b = gm.BubbleBatch(R0, amp, ph, fr)
+    base = b.run(p, BD_GM_DP45, 4e-6, 1e-13)
+    for block, chunk in ((64, 1), (128, 16), (256, 64)):
+        o = b.run(p, BD_GM_DP45, 4e-6, 1e-13, block=block, chunk=chunk)
+        torch.cuda.synchronize()
+        assert torch.equal(o["Rmax"], base["Rmax"])
+        assert torch.equal(o["n_acc"], base["n_acc"])
