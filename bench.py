@@ -58,11 +58,15 @@ def cpu_baseline(n: int, budget_s: float) -> dict:
 
 
 def read_traffic(profile: Path) -> float | None:
+    """HBM bytes per launch of the interior kernel from the committed counter
+    summary (line 'hbm_bytes_per_launch: N'), or None."""
     try:
-        d = json.loads(profile.read_text())
-        return float(d["hbm_bytes_per_launch"])
-    except Exception:
-        return None
+        for line in profile.read_text().splitlines():
+            if line.startswith("hbm_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def main() -> int:
@@ -74,6 +78,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="first copy a buffer of this size 3 times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the multi-GPU z-slab decomposition is not built "
@@ -89,6 +95,13 @@ def main() -> int:
         print("no GPU visible", file=sys.stderr)
         return 3
     _ffi.load()
+    if args.calib_copy_mb:
+        src = torch.ones(args.calib_copy_mb * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     n, K, W = args.n, args.steps, args.warmup
     p = presets.tone_burst_192(n)
     focus = presets.tone_burst_focus(p, n)
@@ -162,7 +175,7 @@ def main() -> int:
         except Exception as exc:  # the baseline is reported, never required
             log(f"cpu baseline failed: {exc}")
 
-    traffic = read_traffic(ROOT / "profiles" / f"westervelt_march_{n}_pmc.json")
+    traffic = read_traffic(ROOT / "profiles" / f"westervelt_march_{n}_pmc.txt")
     out = {
         "metric": "Westervelt Mcell-steps/s & %HBM roofline; Gilmore bubble-steps/s; vs FP64 plain",
         "value": round(value, 2),

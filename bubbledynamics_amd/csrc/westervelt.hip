@@ -273,8 +273,8 @@ __global__ void energy_finalize_kernel(const double* epart, double* energy, int 
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
-constexpr int kDefaultVariant = 2;
-constexpr int kDefaultBlocks = 2048;
+constexpr int kDefaultVariant = 0;
+constexpr int kDefaultBlocks = 1024;
 
 // Variant / target block count for the interior kernel; BD_WV_VARIANT and
 // BD_WV_BLOCKS override the defaults (used by the tile-shape sweep).
