@@ -89,6 +89,18 @@ class GilmoreBatch(C.Structure):
                 ("n_acc", C.c_void_p), ("n_rej", C.c_void_p), ("reserved", C.c_double * 4)]
 
 
+class BioheatParams(C.Structure):
+    """Mirror of bd_bioheat_params (T:30-45 thermal block plus Pennes extension)."""
+    _fields_ = [(n, C.c_double) for n in (
+        "water_tissue", "dxy", "dz", "dt_T", "kdt_dxy", "kdt_dz", "kdt_dxy_water", "kdt_dz_water",
+        "dt_rho_ct", "dt_rho_ct_water", "perf", "T_art", "T_off")] + [("reserved", C.c_double * 8)]
+
+    def copy(self) -> "BioheatParams":
+        o = BioheatParams()
+        C.pointer(o)[0] = self
+        return o
+
+
 BD_GM_TAB_REFERENCE = 0
 BD_GM_TAB_DP1980 = 1
 BD_GM_RK4 = 0
@@ -121,6 +133,10 @@ SIGNATURES: dict[str, tuple] = {
     "bd_westervelt_plan_destroy": (_I, [_VP]),
     "bd_westervelt_plan_interior_only": (_I, [_VP, _VP]),
     "bd_westervelt_plan_interior_cells": (C.c_longlong, [_VP]),
+    "bd_temperature_square": (_I, [C.POINTER(BioheatParams), _I, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _I, _VP]),
+    "bd_bioheat_step": (_I, [C.POINTER(BioheatParams), _I, _VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _VP]),
+    "bd_spiral_reference": (_I, [_D, _D, _D, _D, _D, _D, _D, _I, C.POINTER(_I), C.POINTER(_I), _I]),
+    "bd_spiral_dwell": (_I, [_I, _D, _D, C.POINTER(_I), C.POINTER(_I)]),
     "bd_gilmore_reference_params": (None, [C.POINTER(GilmoreParams)]),
     "bd_gilmore_threshold_sweep": (_I, [C.POINTER(GilmoreParams), _VP, _VP, _I, _I, _I, _VP]),
     "bd_gilmore_workspace_bytes": (C.c_size_t, []),

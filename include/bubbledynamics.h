@@ -172,6 +172,53 @@ int bd_westervelt_plan_interior_only(bd_westervelt_plan* plan, void* stream);
 long long bd_westervelt_plan_interior_cells(const bd_westervelt_plan* plan);
 
 /* ======================================================================== */
+/*  Bioheat (heat conduction + acoustic source, CEM43 thermal dose)         */
+/* ======================================================================== */
+
+/* The thermal part of the reference's __constant__ block (T:35-45) plus the
+ * Pennes extension; perf = 0 and T_off = 0 reproduce the reference. */
+typedef struct bd_bioheat_params {
+    double water_tissue;        /* T:30: planes with iz*dz < water_tissue use the water set */
+    double dxy, dz, dt_T;       /* T:35-37 */
+    double kdt_dxy, kdt_dz, kdt_dxy_water, kdt_dz_water;   /* T:40-43: k dt / h^2 */
+    double dt_rho_ct, dt_rho_ct_water;                     /* T:44-45: dt / (rho c_t) */
+    double perf;                /* Pennes: w_b c_b dt / (rho c_t) in tissue (0 = none) */
+    double T_art;               /* arterial temperature [K] */
+    double T_off;               /* extended step only: fields hold T - T_off [K] */
+    double reserved[8];
+} bd_bioheat_params;
+
+/* Drop-in for Temperature_square<<<>>> (T:52-105): 7-point explicit update of
+ * the interior, shifted acoustic source Q[ix-shift_x, iy-shift_y, iz] (0
+ * outside), CEM43 dose TD += R^(43-T_c) dt_T.  use_p=1 reads T_p and writes
+ * T_n; use_p=0 the reverse. */
+int bd_temperature_square(const bd_bioheat_params* prm, int dtype, void* T_n, void* T_p,
+                          const void* Q, void* TD, int Nxy, int Nz, int use_p,
+                          int shift_x, int shift_y, void* stream);
+
+/* Extended step: T_out <- update(T_in), Pennes perfusion, fields stored as
+ * T - T_off, and with TD_comp != NULL a Kahan-compensated dose sum. */
+int bd_bioheat_step(const bd_bioheat_params* prm, int dtype, void* T_out, const void* T_in,
+                    const void* Q, void* TD, void* TD_comp, int Nxy, int Nz,
+                    int shift_x, int shift_y, void* stream);
+
+/* The reference's continuous square-spiral motion (T:141-173, T:323-409):
+ * per time step the integer source shift (shift_x, shift_y).  start is the
+ * initial focus coordinate (x = y = start; the reference uses
+ * -0.085 + 425*dxy, T:331-332).  Writes at most cap entries and returns the
+ * number of steps (77,474 for the reference's 13 lines), or -1 on bad
+ * arguments. */
+int bd_spiral_reference(double speed, double scale, double speed_init, double t_acc,
+                        double dt, double dxy, double start, int lines, int* shift_x,
+                        int* shift_y, int cap);
+
+/* Dwell spiral: n_side x n_side focus positions at pitch (m) on the grid
+ * spacing dxy, visited as a square spiral from the centre (legs 1,1,2,2,...
+ * with the reference's leg order: +x, +y, -x, -y, ...).  Writes n_side^2
+ * integer shifts (rounded to the nearest cell). */
+int bd_spiral_dwell(int n_side, double pitch, double dxy, int* shift_x, int* shift_y);
+
+/* ======================================================================== */
 /*  Gilmore bubble dynamics in viscoelastic tissue                          */
 /* ======================================================================== */
 

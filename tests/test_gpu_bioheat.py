@@ -1,0 +1,105 @@
+"""GPU parity of the bioheat kernel against the FP64 oracle (through the C ABI)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from bubbledynamics_amd import _ffi  # noqa: E402
+from bubbledynamics_amd import bioheat as bh  # noqa: E402
+from bubbledynamics_amd._ffi import BD_F32, BD_F64  # noqa: E402
+
+
+def dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
+
+
+def vp(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def two_layer(n, nz):
+    # the reference's water/tissue split (T:79) at mid-depth, coarse grid
+    p, _, _ = bh.reference_params()
+    p.water_tissue = (nz // 2) * p.dz + 1e-9
+    return p
+
+
+@pytest.mark.parametrize("use_p", [1, 0])
+def test_temperature_square_dropin_fp64(orc, use_p):
+    rng = np.random.default_rng(5)
+    n, nz = 40, 36
+    p = two_layer(n, nz)
+    shp = (nz, n, n)
+    Tn = 300.0 + rng.uniform(0, 20, shp)
+    Tp = 300.0 + rng.uniform(0, 20, shp)
+    Q = rng.uniform(0, 1e8, shp)
+    TD = rng.uniform(0, 10, shp)
+    dTn, dTp, dQ, dTD = dev(Tn), dev(Tp), dev(Q), dev(TD)
+    _ffi.call("bd_temperature_square", C.byref(p), BD_F64, vp(dTn), vp(dTp), vp(dQ), vp(dTD), n, nz,
+              use_p, 3, -2, None)
+    torch.cuda.synchronize()
+    orc.temperature_square(p, Tn, Tp, Q, TD, n, nz, use_p, 3, -2)
+    out_ref = Tn if use_p else Tp
+    out = (dTn if use_p else dTp).cpu().numpy()
+    np.testing.assert_allclose(out, out_ref, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(dTD.cpu().numpy(), TD, rtol=1e-13)
+
+
+def test_pennes_fp32_offset_kahan_vs_oracle(orc):
+    # 300 explicit substeps with perfusion and a focal source moved along 6 dwell points:
+    # FP32 fields stored as T - T_art and a compensated dose against the FP64 oracle
+    n = 48
+    p = bh.pennes_params()
+    rng = np.random.default_rng(2)
+    z, y, x = np.meshgrid(*(np.arange(n) - n / 2,) * 3, indexing="ij")
+    Q = 3e7 * np.exp(-(x * x + y * y) / 8.0 - z * z / 30.0)
+    T0 = p.T_art
+    sx, sy = bh.spiral_dwell(3, 8 * 2e-4, 2e-4)
+    run = bh.BioheatRun(p, n, n, dev(Q, torch.float32), T0, dtype=torch.float32, kahan=True, offset=True)
+    plain = bh.BioheatRun(p, n, n, dev(Q, torch.float32), T0, dtype=torch.float32, kahan=False, offset=False)
+    A = np.full((n, n, n), T0); B = A.copy(); TD = np.zeros_like(A); TDc = np.zeros_like(A)
+    use_p = True
+    for k in range(6):
+        run.step(int(sx[k]), int(sy[k]), 50)
+        plain.step(int(sx[k]), int(sy[k]), 50)
+        for _ in range(50):
+            orc.temperature_square(p, A, B, Q, TD, n, n, use_p, int(sx[k]), int(sy[k]), TDc=TDc)
+            use_p = not use_p
+    torch.cuda.synchronize()
+    T_ref = A if not use_p else B
+    dT = run.temperature().cpu().numpy() - T_ref
+    rise = (T_ref - T0).max()
+    dose_ref = TD - TDc
+    print(f"peak rise {rise:.2f} K; FP32(offset) max |dT| {np.abs(dT).max():.2e} K; "
+          f"FP32(kelvin) max |dT| {np.abs(plain.temperature().cpu().numpy() - T_ref).max():.2e} K")
+    assert rise > 5.0
+    assert np.abs(dT).max() <= 2e-4 * rise
+    d = run.dose().cpu().numpy()
+    sel = dose_ref > 1e-3 * dose_ref.max()
+    rel = np.abs(d[sel] / dose_ref[sel] - 1)
+    print(f"dose rel err max {rel.max():.2e}")
+    assert rel.max() <= 1e-3
+
+
+def test_offset_storage_is_exact_change_of_variables(orc):
+    # FP64: storing T - T_art gives the same temperatures as kelvin storage
+    n = 24
+    p = bh.pennes_params()
+    Q = np.zeros((n, n, n)); Q[12, 12, 12] = 5e8
+    a = bh.BioheatRun(p, n, n, dev(Q), p.T_art + 1.0, dtype=torch.float64, kahan=False, offset=True)
+    b = bh.BioheatRun(p, n, n, dev(Q), p.T_art + 1.0, dtype=torch.float64, kahan=False, offset=False)
+    a.step(1, 1, 40)
+    b.step(1, 1, 40)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(a.temperature().cpu().numpy(), b.temperature().cpu().numpy(), rtol=1e-13)
+
+
+def test_unstable_step_rejected():
+    p = bh.pennes_params(substeps=1)
+    t = torch.zeros(8 ** 3, dtype=torch.float32, device="cuda")
+    rc = _ffi.load().bd_bioheat_step(C.byref(p), BD_F32, vp(t), vp(t.clone()), vp(t), vp(t), None, 8, 8, 0, 0, None)
+    assert rc == _ffi.BD_ERR_UNSTABLE

@@ -1,0 +1,133 @@
+"""Pins of the FP64 bioheat oracle (CPU only) and of the host-side trajectory
+generators of the library against the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+from bubbledynamics_amd import bioheat as bh
+
+
+def homog(n=24, r=0.1):
+    p = bh.pennes_params(dx=2e-4, dt=0.05, substeps=2, w_b=0.0)
+    p.kdt_dxy = p.kdt_dz = p.kdt_dxy_water = p.kdt_dz_water = r
+    return p
+
+
+def run(orc, p, T, Q, n_steps, sx=0, sy=0, TD=None, TDc=None):
+    nz, ny, nx = T.shape
+    a, b = T.copy(), T.copy()
+    TD = np.zeros_like(T) if TD is None else TD
+    use_p = True
+    for _ in range(n_steps):
+        orc.temperature_square(p, a, b, Q, TD, nx, nz, use_p, sx, sy, TDc=TDc)
+        use_p = not use_p
+    return (a if not use_p else b), TD
+
+
+def test_eigenmode_decays_by_exact_factor(orc):
+    # T - T0 = A sin(i th) sin(j th) sin(k th_z): each step multiplies A by
+    # 1 - 4r sin^2(th/2) - 4r sin^2(th/2) - 4r_z sin^2(th_z/2) (T:85-96)
+    n, r, steps = 20, 0.1, 25
+    p = homog(n, r)
+    th = math.pi / (n - 1)
+    i = np.arange(n)
+    mode = np.sin(i * th)[:, None, None] * np.sin(i * th)[None, :, None] * np.sin(i * th)[None, None, :]
+    T0 = 310.0
+    T = T0 + 2.0 * mode
+    out, _ = run(orc, p, T, np.zeros_like(T), steps)
+    lam = 1 - 3 * 4 * r * math.sin(th / 2) ** 2
+    np.testing.assert_allclose(out - T0, 2.0 * lam ** steps * mode, atol=1e-11)
+
+
+def test_constant_temperature_dose_closed_form(orc):
+    # CEM43 (T:99-102): TD = R^(43-T) t with R = 0.5 above 43 C, 0.25 below
+    n, steps = 8, 7
+    for Tc in (45.0, 40.0, 43.0):
+        p = homog(n)
+        T = np.full((n, n, n), Tc + 273.15)
+        _, TD = run(orc, p, T, np.zeros_like(T), steps)
+        R = 0.5 if Tc >= 43 else 0.25
+        expect = steps * R ** (43 - Tc) * p.dt_T
+        np.testing.assert_allclose(TD[1:-1, 1:-1, 1:-1], expect, rtol=1e-14)
+        assert np.all(TD[0] == 0) and np.all(TD[:, :, -1] == 0)   # boundary never updated
+
+
+def test_source_heats_and_shift_moves_it(orc):
+    # uniform field: one step adds dt/(rho c) Q; the source appears shifted by (sx, sy)
+    n = 12
+    p = homog(n)
+    T = np.full((n, n, n), 300.0)
+    Q = np.zeros_like(T)
+    Q[5, 4, 3] = 1e7
+    out, _ = run(orc, p, T, Q, 1, sx=2, sy=-1)
+    d = out - 300.0
+    assert d[5, 3, 5] == pytest.approx(p.dt_rho_ct * 1e7, rel=1e-12)    # (iz, iy-1, ix+2)
+    d[5, 3, 5] = 0.0
+    assert np.abs(d).max() == 0.0
+    # shifted completely out of the grid: no heating
+    out, _ = run(orc, p, T, Q, 1, sx=20, sy=0)
+    assert np.abs(out - 300.0).max() == 0.0
+
+
+def test_perfusion_relaxes_to_pennes_equilibrium(orc):
+    # uniform field: T_{n+1} - T_eq = (1 - perf) (T_n - T_eq), T_eq = T_art + dq Q / perf
+    n, steps = 40, 12
+    p = bh.pennes_params()
+    T0 = 300.0
+    T = np.full((n, n, n), T0)
+    Q = np.full_like(T, 2e6)
+    out, _ = run(orc, p, T, Q, steps)
+    Teq = p.T_art + p.dt_rho_ct * 2e6 / p.perf
+    expect = Teq + (1 - p.perf) ** steps * (T0 - Teq)
+    c = n // 2
+    assert out[c, c, c] == pytest.approx(expect, rel=1e-13)
+    # without perfusion (the reference) the same cell just heats linearly
+    q = p.copy()
+    q.perf = 0.0
+    out0, _ = run(orc, q, T, Q, steps)
+    assert out0[c, c, c] == pytest.approx(T0 + steps * p.dt_rho_ct * 2e6, rel=1e-13)
+
+
+def test_reference_coefficients():
+    # T:212-217 with the reference constants
+    p, nxy, nz = bh.reference_params()
+    assert (nxy, nz) == (851, 2001)
+    assert p.kdt_dxy == pytest.approx(3.27e-3, rel=5e-3)
+    assert p.kdt_dz == pytest.approx(1.31e-2, rel=5e-3)
+    assert p.kdt_dxy_water == pytest.approx(3.57e-3, rel=5e-3)
+    assert p.kdt_dz_water == pytest.approx(1.43e-2, rel=5e-3)
+
+
+def test_pennes_substeps_needed():
+    # a single 0.05 s step breaks the explicit limit r <= 1/6; two substeps do not
+    one = bh.pennes_params(substeps=1)
+    two = bh.pennes_params(substeps=2)
+    assert 1 - 6 * one.kdt_dxy < 0
+    assert 1 - 6 * two.kdt_dxy - two.perf > 0
+    assert one.kdt_dxy == pytest.approx(0.1806, rel=1e-3)
+
+
+def test_reference_spiral_library_matches_oracle(orc):
+    # T:141-173, T:323-409: the library's host generator against the oracle,
+    # integer sequence for integer sequence
+    sx, sy = bh.spiral_reference()
+    ox, oy = orc.spiral_reference()
+    assert len(sx) == 77474 == 49 * 1501 + 13 * 2 * 151 - 1
+    np.testing.assert_array_equal(sx, ox)
+    np.testing.assert_array_equal(sy, oy)
+    # the focus moves by one cell at a time, along x and y alternately
+    assert np.all(np.abs(np.diff(sx)) <= 1) and np.all(np.abs(np.diff(sy)) <= 1)
+    assert not np.any((np.diff(sx) != 0) & (np.diff(sy) != 0))
+
+
+def test_dwell_spiral():
+    sx, sy = bh.spiral_dwell(7, 8e-3, 2e-4)
+    pts = list(zip(sx.tolist(), sy.tolist()))
+    assert pts[0] == (0, 0) and len(set(pts)) == 49
+    cells = [round(k * (8e-3 / 6) / 2e-4) for k in range(-3, 4)]
+    assert sorted(set(sx.tolist())) == cells and sorted(set(sy.tolist())) == cells
+    # consecutive points are neighbours on the 7 x 7 lattice; first legs +x, +y, -x, -x
+    gx = np.array([cells.index(v) for v in sx]); gy = np.array([cells.index(v) for v in sy])
+    assert np.all(np.abs(np.diff(gx)) + np.abs(np.diff(gy)) == 1)
+    assert (gx[1] - gx[0], gy[2] - gy[1], gx[3] - gx[2], gx[4] - gx[3]) == (1, 1, -1, -1)
