@@ -101,6 +101,12 @@ class BioheatParams(C.Structure):
         return o
 
 
+class AdjointObjective(C.Structure):
+    _fields_ = [("cx", C.c_int32), ("cy", C.c_int32), ("cz", C.c_int32), ("pad0", C.c_int32),
+                ("radius", C.c_double), ("w_target", C.c_double), ("w_outside", C.c_double),
+                ("reserved", C.c_double * 4)]
+
+
 BD_GM_TAB_REFERENCE = 0
 BD_GM_TAB_DP1980 = 1
 BD_GM_RK4 = 0
@@ -133,6 +139,8 @@ SIGNATURES: dict[str, tuple] = {
     "bd_westervelt_plan_destroy": (_I, [_VP]),
     "bd_westervelt_plan_interior_only": (_I, [_VP, _VP]),
     "bd_westervelt_plan_interior_cells": (C.c_longlong, [_VP]),
+    "bd_westervelt_adjoint": (_I, [_PP, _I, _I, _I, _I, _I, _VP, _VP, C.POINTER(AdjointObjective),
+                                   C.POINTER(_D), _VP, _VP, C.POINTER(_D), _VP]),
     "bd_temperature_square": (_I, [C.POINTER(BioheatParams), _I, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _I, _VP]),
     "bd_bioheat_step": (_I, [C.POINTER(BioheatParams), _I, _VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _VP]),
     "bd_spiral_reference": (_I, [_D, _D, _D, _D, _D, _D, _D, _I, C.POINTER(_I), C.POINTER(_I), _I]),

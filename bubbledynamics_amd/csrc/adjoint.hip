@@ -1,0 +1,411 @@
+// Discrete adjoint of the Westervelt step for phased-array gradients (gfx950).
+//
+// Forward model (the library's own step, full domain, boundary layers held at
+// zero, sponge G, hard array source on plane iz=2):
+//   E^n   = 2p^n - p^{n-1} + kL p^n + dlt D3^n + bet N^n + s (R1^n + R2^n)
+//   p^{n+1} = G E^n (interior),  p^{n+1} = A_e sin(w t_{n+1} + phi_e) (source cells)
+//   D3^n  = 5p^n - 18p^{n-1} + 24p^{n-2} - 14p^{n-3} + 3p^{n-4}
+//   N^n   = A^2/4 + p^n B,  A = 3p^n - 4p^{n-1} + p^{n-2},  B = 2p^n - 5p^{n-1} + 4p^{n-2} - p^{n-3}
+//   R_v^n = a_v D3^n + b_v (4 R_v^{n-1} - R_v^{n-2})
+// Objective J = sum_{n=1..Nt} [ -wT sum_target (p^n)^2 + wO sum_outside (p^n)^2 ].
+//
+// Backward step n (lam^{n+1} complete -> lam^n), with w = G lam^{n+1} on interior cells:
+//   rho_v^n = s w + 4 b_v rho_v^{n+1} - b_v rho_v^{n+2}
+//   d^n = dlt w + a_1 rho_1^n + a_2 rho_2^n,   e^n = bet w
+//   lam^n  = acc[n] + 2w + 5d + e (3A/2 + B + 2p^n) + (kL)^T w + dJ/dp^n
+//   acc[n-1] += -w - 18d + e(-2A - 5p^n);  acc[n-2] += 24d + e(A/2 + 4p^n)
+//   acc[n-3] += -14d - e p^n;              acc[n-4]  = 3d
+// At a source cell lam^n = (kL)^T w = kz (270 w(3) - 27 w(4) + 2 w(5)) / 180, and
+// dJ/dA_e = sum_n sum_{c in e} lam^n_c sin(w t_n + phi_e), dJ/dphi_e = ... A_e cos(...).
+// The forward levels a backward segment needs are recomputed from checkpoints
+// taken every K steps (K >= n_steps keeps every level instead).
+#include "common.h"
+#include "wv_march.h"
+#include "wv_internal.h"
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace bd {
+
+template <typename T>
+struct AdjArgs {
+    const T* lam1; T* lam0;
+    const T* p0; const T* p1; const T* p2; const T* p3;
+    const T* acc0; T* acc1; T* acc2; T* acc3; T* acc4;
+    const T* r1a; const T* r1b; T* r1o;
+    const T* r2a; const T* r2b; T* r2o;
+    T* lsrc;                          // lam^n on plane iz=2 (N*N), nullable
+    T kxy, kz, dlt, bet, s, a1, b1, a2, b2;
+    int N, Nz, sponge_w;
+    double sponge_alpha;
+    int ocx, ocy, ocz;
+    double or2, wT, wO;
+    int ntx, nty, nzc, zchunk, nblocks;
+};
+
+template <typename T>
+__device__ __forceinline__ T sponge_g(int ix, int iy, int iz, int N, int Nz, int W, double alpha) {
+    if (W <= 0) return T(1);
+    int dx = min(N - 1 - ix, ix), dy = min(N - 1 - iy, iy), dz = Nz - 1 - iz;
+    if (!(dx < W || dy < W || dz < W)) return T(1);
+    T sp = T(0);
+    if (dx < W) { const T u = T(alpha) * T(W - dx); sp += u * u; }
+    if (dy < W) { const T u = T(alpha) * T(W - dy); sp += u * u; }
+    if (dz < W) { const T u = T(alpha) * T(W - dz); sp += u * u; }
+    return exp(-sp);
+}
+
+constexpr int kAX = 64, kAY = 4;
+
+template <typename T>
+__global__ void __launch_bounds__(kAX * kAY) adj_step_kernel(AdjArgs<T> a) {
+    __shared__ T s[2][kAY + 6][kAX + 6];
+    const int lb = xcd_remap((int)blockIdx.x, a.nblocks);
+    const int zc = lb % a.nzc, t2 = lb / a.nzc;
+    const int bx = t2 % a.ntx, by = t2 / a.ntx;
+    const int tx = threadIdx.x % kAX, ty = threadIdx.x / kAX;
+    const int N = a.N, Nz = a.Nz;
+    const size_t plane = (size_t)N * N;
+    const int ix = 3 + bx * kAX + tx, iy = 3 + by * kAY + ty;
+    const bool act = ix <= N - 4 && iy <= N - 4;
+    const int z0 = 3 + zc * a.zchunk;
+    const int z1 = min(z0 + a.zchunk, Nz - 3);
+    // w = G lam^{n+1} on interior cells, 0 elsewhere
+    auto wval = [&](int x, int y, int z) -> T {
+        if (x < 3 || x > N - 4 || y < 3 || y > N - 4 || z < 3 || z > Nz - 4) return T(0);
+        return sponge_g<T>(x, y, z, N, Nz, a.sponge_w, a.sponge_alpha) * a.lam1[(size_t)z * plane + (size_t)y * N + x];
+    };
+    T q[7];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q[k] = (ix < N && iy < N) ? wval(ix, iy, z0 - 3 + k) : T(0);
+    q[6] = T(0);
+    if (zc == 0 && act && a.lsrc) {
+        // source plane iz=2: only the z stencil reaches interior cells
+        a.lsrc[(size_t)iy * N + ix] = a.kz * (T(270.0) * q[3] - T(27.0) * q[4] + T(2.0) * q[5]) / T(180.0);
+    }
+    const int hx = 3 + bx * kAX - 3 + tx;   // left halo column of this thread (tx < 3)
+    const int hy = 3 + by * kAY - 3 + ty;   // bottom halo row (ty < 3)
+    for (int z = z0; z < z1; ++z) {
+        const int b = (z - z0) & 1;
+        q[6] = (ix < N && iy < N) ? wval(ix, iy, z + 3) : T(0);
+        s[b][ty + 3][tx + 3] = q[3];
+        if (tx < 3) s[b][ty + 3][tx] = (iy < N) ? wval(hx, iy, z) : T(0);
+        if (tx >= kAX - 3) s[b][ty + 3][tx + 6] = (iy < N && ix + 3 < N) ? wval(ix + 3, iy, z) : T(0);
+        if (ty < 3) s[b][ty][tx + 3] = (ix < N) ? wval(ix, hy, z) : T(0);
+        if (ty >= kAY - 3) s[b][ty + 6][tx + 3] = (ix < N && iy + 3 < N) ? wval(ix, iy + 3, z) : T(0);
+        __syncthreads();
+        if (act) {
+            const size_t i = (size_t)z * plane + (size_t)iy * N + ix;
+            const T w = q[3];
+            const T* row = &s[b][ty + 3][tx + 3];
+            const T Lx = (T(-490.0) * w + T(270.0) * (row[1] + row[-1]) - T(27.0) * (row[2] + row[-2]) +
+                          T(2.0) * (row[3] + row[-3])) / T(180.0);
+            const T Ly = (T(-490.0) * w + T(270.0) * (s[b][ty + 4][tx + 3] + s[b][ty + 2][tx + 3]) -
+                          T(27.0) * (s[b][ty + 5][tx + 3] + s[b][ty + 1][tx + 3]) +
+                          T(2.0) * (s[b][ty + 6][tx + 3] + s[b][ty][tx + 3])) / T(180.0);
+            const T Lz = (T(-490.0) * w + T(270.0) * (q[4] + q[2]) - T(27.0) * (q[5] + q[1]) +
+                          T(2.0) * (q[6] + q[0])) / T(180.0);
+            const T ST = a.kxy * (Lx + Ly) + a.kz * Lz;
+            const T pn = a.p0[i], pm1 = a.p1[i], pm2 = a.p2[i], pm3 = a.p3[i];
+            const T A = T(3.0) * pn - T(4.0) * pm1 + pm2;
+            const T B = T(2.0) * pn - T(5.0) * pm1 + T(4.0) * pm2 - pm3;
+            const T rho1 = a.s * w + T(4.0) * a.b1 * a.r1a[i] - a.b1 * a.r1b[i];
+            const T rho2 = a.s * w + T(4.0) * a.b2 * a.r2a[i] - a.b2 * a.r2b[i];
+            const T d = a.dlt * w + a.a1 * rho1 + a.a2 * rho2;
+            const T e = a.bet * w;
+            const int dx = ix - a.ocx, dy = iy - a.ocy, dz = z - a.ocz;
+            const bool tgt = (double)(dx * dx + dy * dy + dz * dz) <= a.or2;
+            const T dj = tgt ? T(-2.0 * a.wT) * pn : T(2.0 * a.wO) * pn;
+            a.lam0[i] = a.acc0[i] + T(2.0) * w + T(5.0) * d + e * (T(1.5) * A + B + T(2.0) * pn) + ST + dj;
+            a.acc1[i] = a.acc1[i] + (-w - T(18.0) * d + e * (T(-2.0) * A - T(5.0) * pn));
+            a.acc2[i] = a.acc2[i] + (T(24.0) * d + e * (T(0.5) * A + T(4.0) * pn));
+            a.acc3[i] = a.acc3[i] + (T(-14.0) * d - e * pn);
+            a.acc4[i] = T(3.0) * d;
+            a.r1o[i] = rho1;
+            a.r2o[i] = rho2;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) q[k] = q[k + 1];
+    }
+}
+
+// lam^{Nt} = dJ/dp^{Nt} on interior cells (nothing follows the last level).
+template <typename T>
+__global__ void adj_init_kernel(T* lam, const T* p, int N, int Nz, int ocx, int ocy, int ocz, double or2, double wT,
+                                double wO) {
+    const size_t n = (size_t)N * N * Nz;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int ix = (int)(i % N), iy = (int)((i / N) % N), iz = (int)(i / ((size_t)N * N));
+    T v = T(0);
+    if (ix >= 3 && ix <= N - 4 && iy >= 3 && iy <= N - 4 && iz >= 3 && iz <= Nz - 4) {
+        const int dx = ix - ocx, dy = iy - ocy, dz = iz - ocz;
+        const bool tgt = (double)(dx * dx + dy * dy + dz * dz) <= or2;
+        v = tgt ? T(-2.0 * wT) * p[i] : T(2.0 * wO) * p[i];
+    }
+    lam[i] = v;
+}
+
+// Per-element gradient: one workgroup per element, FP64 accumulation over the
+// element's source cells and all levels n = 1..Nt-1.
+template <typename T>
+__global__ void __launch_bounds__(256) adj_grad_kernel(const T* lsrc, const int* cells, const int* cell_off,
+                                                       const double* amp, const double* phase, const double* times,
+                                                       int N, int Nt, double w, double burst_cycles, double burst_t0,
+                                                       double fr, double* gA, double* gP) {
+    __shared__ double red[4];
+    const int e = blockIdx.x;
+    const int c0 = cell_off[e], c1 = cell_off[e + 1];
+    const size_t plane = (size_t)N * N;
+    double sa = 0.0, sp = 0.0;
+    for (int n = 1 + (int)threadIdx.x; n < Nt; n += blockDim.x) {
+        double S = 0.0;
+        for (int c = c0; c < c1; ++c) S += (double)lsrc[(size_t)n * plane + cells[c]];
+        double tb = times[n];
+        bool on = true;
+        if (burst_cycles > 0) {
+            tb = times[n] - burst_t0;
+            on = !(tb < 0.0 || tb * fr >= burst_cycles);
+        }
+        if (!on) continue;
+        double sn, cs;
+        sincos(w * tb + phase[e], &sn, &cs);
+        sa += S * sn;
+        sp += S * amp[e] * cs;
+    }
+    const double A = block_sum<256>(sa, red);
+    const double P = block_sum<256>(sp, red);
+    if (threadIdx.x == 0) { gA[e] = A; gP[e] = P; }
+}
+
+// J partial sums: a single workgroup, fixed order.
+__global__ void __launch_bounds__(256) adj_sum_kernel(const double* epart, size_t n2, double* out) {
+    __shared__ double red[4];
+    double a = 0.0, b = 0.0;
+    for (size_t i = threadIdx.x; i < n2; i += blockDim.x) { a += epart[2 * i]; b += epart[2 * i + 1]; }
+    const double A = block_sum<256>(a, red);
+    const double B = block_sum<256>(b, red);
+    if (threadIdx.x == 0) { out[0] = A; out[1] = B; }
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    bool alloc(size_t bytes) { return hipMalloc(&p, bytes) == hipSuccess && hipMemset(p, 0, bytes) == hipSuccess; }
+};
+
+template <typename T>
+static int adjoint_t(const bd_westervelt_params* prm, int Nxy, int Nz, int Nt, int K, const double* amp,
+                     const double* phase, const bd_adjoint_objective* obj, double* J, double* gA, double* gP,
+                     double* timings, hipStream_t st) {
+    const int N = Nxy;
+    const size_t plane = (size_t)N * N, NF = plane * Nz, FB = NF * sizeof(T);
+    if (K <= 0 || K > Nt) K = Nt;
+    const int nseg = (Nt + K - 1) / K;
+    const int nb = wv_forward_nblocks(prm, Nxy, Nz);
+    // forward workspace: segment store p^{a-4..a+K}, R rings, checkpoints
+    DevBuf seg, rring, ckp, epart, jsum;
+    if (!seg.alloc(FB * (size_t)(K + 5)) || !rring.alloc(FB * 6) || !epart.alloc(sizeof(double) * 2 * nb * (size_t)Nt) ||
+        !jsum.alloc(sizeof(double) * 2))
+        return BD_ERR_ALLOC;
+    if (nseg > 1 && !ckp.alloc(FB * 9 * (size_t)nseg)) return BD_ERR_ALLOC;
+    T* S = (T*)seg.p;
+    T* Rr = (T*)rring.p;
+    auto Sl = [&](int m, int a0) { return S + (size_t)(m - (a0 - 4)) * NF; };   // p^m in the store of segment a0
+    auto R1 = [&](int m) { return Rr + (size_t)(((m % 3) + 3) % 3) * NF; };
+    auto R2 = [&](int m) { return Rr + (size_t)(3 + ((m % 3) + 3) % 3) * NF; };
+    auto CK = [&](int j, int k) { return (T*)ckp.p + ((size_t)j * 9 + k) * NF; };
+    std::vector<double> times((size_t)Nt + 1);
+    {
+        double t = 0.0;
+        for (int k = 0; k <= Nt; ++k) { times[k] = t; t = t + prm->dt; }
+    }
+    WvObjective ob{obj->cx, obj->cy, obj->cz, obj->radius * obj->radius};
+    auto step = [&](int m, int a0, double* ep) -> int {
+        WvStepPtrs q;
+        q.pn = Sl(m + 1, a0); q.p = Sl(m, a0); q.pm1 = Sl(m - 1, a0); q.pm2 = Sl(m - 2, a0);
+        q.pm3 = Sl(m - 3, a0); q.pm4 = Sl(m - 4, a0);
+        q.r1 = R1(m); q.r1m1 = R1(m - 1); q.r1m2 = R1(m - 2);
+        q.r2 = R2(m); q.r2m1 = R2(m - 1); q.r2m2 = R2(m - 2);
+        return wv_forward_step(prm, sizeof(T) == 4 ? BD_F32 : BD_F64, Nxy, Nz, q, m, times[(size_t)m + 1], amp, phase,
+                               ep, &ob, st);
+    };
+    auto save_ck = [&](int j, int a0) -> int {  // state at step a0 (store of segment a0) into checkpoint j
+        for (int k = 0; k < 5; ++k) BD_HIP_CHECK(hipMemcpyAsync(CK(j, k), Sl(a0 - k, a0), FB, hipMemcpyDeviceToDevice, st));
+        BD_HIP_CHECK(hipMemcpyAsync(CK(j, 5), R1(a0 - 1), FB, hipMemcpyDeviceToDevice, st));
+        BD_HIP_CHECK(hipMemcpyAsync(CK(j, 6), R1(a0 - 2), FB, hipMemcpyDeviceToDevice, st));
+        BD_HIP_CHECK(hipMemcpyAsync(CK(j, 7), R2(a0 - 1), FB, hipMemcpyDeviceToDevice, st));
+        BD_HIP_CHECK(hipMemcpyAsync(CK(j, 8), R2(a0 - 2), FB, hipMemcpyDeviceToDevice, st));
+        return BD_OK;
+    };
+    auto load_ck = [&](int j, int a0) -> int {  // checkpoint j (state at a0) into the store and R rings
+        for (int k = 0; k < 5; ++k) BD_HIP_CHECK(hipMemcpyAsync(Sl(a0 - k, a0), CK(j, k), FB, hipMemcpyDeviceToDevice, st));
+        BD_HIP_CHECK(hipMemcpyAsync(R1(a0 - 1), CK(j, 5), FB, hipMemcpyDeviceToDevice, st));
+        BD_HIP_CHECK(hipMemcpyAsync(R1(a0 - 2), CK(j, 6), FB, hipMemcpyDeviceToDevice, st));
+        BD_HIP_CHECK(hipMemcpyAsync(R2(a0 - 1), CK(j, 7), FB, hipMemcpyDeviceToDevice, st));
+        BD_HIP_CHECK(hipMemcpyAsync(R2(a0 - 2), CK(j, 8), FB, hipMemcpyDeviceToDevice, st));
+        return BD_OK;
+    };
+    auto now = [&]() {
+        (void)hipStreamSynchronize(st);
+        return std::chrono::steady_clock::now();
+    };
+    auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+
+    // ---- forward pass: objective, checkpoints, last segment kept ----------
+    auto t0 = now();
+    for (int j = 0; j < nseg; ++j) {
+        const int a0 = j * K, b = std::min(Nt, a0 + K);
+        if (j > 0) {
+            // shift the last five levels to the head of the store (ascending
+            // destination slots: a source slot is read before anything lands on it)
+            for (int k = 4; k >= 0; --k)
+                BD_HIP_CHECK(hipMemcpyAsync(Sl(a0 - k, a0), Sl(a0 - k, a0 - K), FB, hipMemcpyDeviceToDevice, st));
+        }
+        if (nseg > 1 && j > 0) { int rc = save_ck(j, a0); if (rc) return rc; }
+        for (int m = a0; m < b; ++m) { int rc = step(m, a0, (double*)epart.p); if (rc) return rc; }
+    }
+    hipLaunchKernelGGL(adj_sum_kernel, dim3(1), dim3(256), 0, st, (const double*)epart.p, (size_t)nb * Nt,
+                       (double*)jsum.p);
+    BD_LAUNCH_CHECK();
+    double js[2];
+    BD_HIP_CHECK(hipMemcpyAsync(js, jsum.p, sizeof(js), hipMemcpyDeviceToHost, st));
+    BD_HIP_CHECK(hipStreamSynchronize(st));
+    *J = -obj->w_target * js[0] + obj->w_outside * js[1];
+    auto t1 = now();
+    if (timings) { timings[0] = ms(t0, t1); timings[1] = 0.0; timings[2] = 0.0; }
+    if (!gA || !gP) return BD_OK;
+
+    // ---- backward pass ----------------------------------------------------
+    DevBuf lam, acc, rho, lsrc;
+    if (!lam.alloc(FB * 2) || !acc.alloc(FB * 5) || !rho.alloc(FB * 6) || !lsrc.alloc(plane * sizeof(T) * (size_t)(Nt + 1)))
+        return BD_ERR_ALLOC;
+    T* L = (T*)lam.p;
+    T* Ac = (T*)acc.p;
+    T* Rh = (T*)rho.p;
+    auto Lv = [&](int m) { return L + (size_t)(m & 1) * NF; };
+    auto Av = [&](int m) { return Ac + (size_t)(((m % 5) + 5) % 5) * NF; };
+    auto Rh1 = [&](int m) { return Rh + (size_t)(((m % 3) + 3) % 3) * NF; };
+    auto Rh2 = [&](int m) { return Rh + (size_t)(3 + ((m % 3) + 3) % 3) * NF; };
+    const WvCoef<T> cf = [&]() {
+        WvCoef<T> c;
+        const double dt = prm->dt;
+        c.kxy[1] = (T)std::pow(prm->c_inf_tissue * dt / prm->dxy, 2);
+        c.kz[1] = (T)std::pow(prm->c_inf_tissue * dt / prm->dz, 2);
+        c.dlt[1] = (T)(prm->delta_tissue / (2.0 * dt * std::pow(prm->c_inf_tissue, 2)));
+        c.bet[1] = (T)(2.0 * prm->beta_tissue / (prm->rho_tissue * std::pow(prm->c_inf_tissue, 2)));
+        c.cdt2 = (T)std::pow(prm->c_inf_tissue * dt, 2);
+        const double two_dt3 = 2.0 * std::pow(dt, 3), ct3 = std::pow(prm->c_inf_tissue, 3);
+        c.a1 = (T)((2.0 * dt * 2.0 * prm->c1 * prm->tau1) / (two_dt3 * (2.0 * dt + 3.0 * prm->tau1) * ct3));
+        c.b1 = (T)(prm->tau1 / (2.0 * dt + 3.0 * prm->tau1));
+        c.a2 = (T)((2.0 * dt * 2.0 * prm->c2 * prm->tau2) / (two_dt3 * (2.0 * dt + 3.0 * prm->tau2) * ct3));
+        c.b2 = (T)(prm->tau2 / (2.0 * dt + 3.0 * prm->tau2));
+        return c;
+    }();
+    AdjArgs<T> A;
+    A.kxy = cf.kxy[1]; A.kz = cf.kz[1]; A.dlt = cf.dlt[1]; A.bet = cf.bet[1]; A.s = cf.cdt2;
+    A.a1 = cf.a1; A.b1 = cf.b1; A.a2 = cf.a2; A.b2 = cf.b2;
+    A.N = N; A.Nz = Nz; A.sponge_w = prm->sponge_cells; A.sponge_alpha = prm->sponge_alpha;
+    A.ocx = obj->cx; A.ocy = obj->cy; A.ocz = obj->cz; A.or2 = obj->radius * obj->radius;
+    A.wT = obj->w_target; A.wO = obj->w_outside;
+    A.ntx = (N - 6 + kAX - 1) / kAX;
+    A.nty = (N - 6 + kAY - 1) / kAY;
+    {
+        const int nzin = Nz - 6;
+        int nzc = std::max(1, 1024 / (A.ntx * A.nty));
+        int zchunk = std::max(std::min(8, nzin), (nzin + nzc - 1) / nzc);
+        A.nzc = (nzin + zchunk - 1) / zchunk;
+        A.zchunk = zchunk;
+        A.nblocks = A.ntx * A.nty * A.nzc;
+    }
+    double t_recompute = 0.0, t_backward = 0.0;
+    {
+        // lam^{Nt}
+        const int last_a0 = (nseg - 1) * K;
+        hipLaunchKernelGGL((adj_init_kernel<T>), dim3((unsigned)((NF + 255) / 256)), dim3(256), 0, st, Lv(Nt),
+                           (const T*)Sl(Nt, last_a0), N, Nz, obj->cx, obj->cy, obj->cz, A.or2, obj->w_target,
+                           obj->w_outside);
+        BD_LAUNCH_CHECK();
+    }
+    for (int j = nseg - 1; j >= 0; --j) {
+        const int a0 = j * K, b = std::min(Nt, a0 + K);
+        auto r0 = now();
+        if (j != nseg - 1) {
+            // recompute p^{a0+1..b} from the checkpoint at a0 (segment 0 starts from rest)
+            if (j == 0) {
+                BD_HIP_CHECK(hipMemsetAsync(Sl(-4, 0), 0, FB * 5, st));
+                BD_HIP_CHECK(hipMemsetAsync(Rr, 0, FB * 6, st));
+            } else {
+                int rc = load_ck(j, a0);
+                if (rc) return rc;
+            }
+            for (int m = a0; m < b; ++m) { int rc = step(m, a0, nullptr); if (rc) return rc; }
+        }
+        auto r1 = now();
+        t_recompute += ms(r0, r1);
+        for (int n = b - 1; n >= std::max(a0, 1); --n) {
+            A.lam1 = Lv(n + 1); A.lam0 = Lv(n);
+            A.p0 = Sl(n, a0); A.p1 = Sl(n - 1, a0); A.p2 = Sl(n - 2, a0); A.p3 = Sl(n - 3, a0);
+            A.acc0 = Av(n); A.acc1 = Av(n - 1); A.acc2 = Av(n - 2); A.acc3 = Av(n - 3); A.acc4 = Av(n - 4);
+            A.r1a = Rh1(n + 1); A.r1b = Rh1(n + 2); A.r1o = Rh1(n);
+            A.r2a = Rh2(n + 1); A.r2b = Rh2(n + 2); A.r2o = Rh2(n);
+            A.lsrc = (T*)lsrc.p + (size_t)n * plane;
+            hipLaunchKernelGGL((adj_step_kernel<T>), dim3(A.nblocks), dim3(kAX * kAY), 0, st, A);
+            BD_LAUNCH_CHECK();
+        }
+        t_backward += ms(r1, now());
+    }
+    // element cell lists (host), then the gradient reduction
+    const int ne = prm->arr_nx * prm->arr_ny;
+    std::vector<std::vector<int>> lists(ne);
+    for (int iy = 3; iy <= N - 4; ++iy)
+        for (int ix = 3; ix <= N - 4; ++ix) {
+            const double X = prm->xy_min + ix * prm->dxy, Y = prm->xy_min + iy * prm->dxy;
+            const double fx = std::floor((X - prm->arr_x0) / prm->arr_pitch);
+            const double fy = std::floor((Y - prm->arr_y0) / prm->arr_pitch);
+            if (fx < 0 || fy < 0 || fx >= prm->arr_nx || fy >= prm->arr_ny) continue;
+            lists[(int)fy * prm->arr_nx + (int)fx].push_back(iy * N + ix);
+        }
+    std::vector<int> cells, off(ne + 1, 0);
+    for (int e = 0; e < ne; ++e) { off[e + 1] = off[e] + (int)lists[e].size(); cells.insert(cells.end(), lists[e].begin(), lists[e].end()); }
+    DevBuf dcells, doff, dtimes;
+    if (!dcells.alloc(sizeof(int) * std::max<size_t>(1, cells.size())) || !doff.alloc(sizeof(int) * off.size()) ||
+        !dtimes.alloc(sizeof(double) * times.size()))
+        return BD_ERR_ALLOC;
+    BD_HIP_CHECK(hipMemcpyAsync(dcells.p, cells.data(), sizeof(int) * cells.size(), hipMemcpyHostToDevice, st));
+    BD_HIP_CHECK(hipMemcpyAsync(doff.p, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, st));
+    BD_HIP_CHECK(hipMemcpyAsync(dtimes.p, times.data(), sizeof(double) * times.size(), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL((adj_grad_kernel<T>), dim3(ne), dim3(256), 0, st, (const T*)lsrc.p, (const int*)dcells.p,
+                       (const int*)doff.p, amp, phase, (const double*)dtimes.p, N, Nt, 2.0 * M_PI * prm->fr,
+                       prm->burst_cycles, prm->burst_t0, prm->fr, gA, gP);
+    BD_LAUNCH_CHECK();
+    BD_HIP_CHECK(hipStreamSynchronize(st));
+    if (timings) { timings[1] = t_recompute; timings[2] = t_backward; }
+    return BD_OK;
+}
+
+}  // namespace bd
+
+using namespace bd;
+
+extern "C" int bd_westervelt_adjoint(const bd_westervelt_params* prm, int dtype, int Nxy, int Nz, int n_steps,
+                                     int checkpoint_every, const double* elem_amp, const double* elem_phase,
+                                     const bd_adjoint_objective* obj, double* J, double* grad_amp,
+                                     double* grad_phase, double* timings_ms, void* stream) {
+    int rc = wv_check_params(prm, dtype, Nxy, Nz);
+    if (rc) return rc;
+    if (!obj || !J || !elem_amp || !elem_phase || n_steps < 1) return BD_ERR_INVALID;
+    if (!prm->full_domain || prm->boundary != BD_BC_ZERO || prm->source != BD_SRC_ARRAY) return BD_ERR_UNSUPPORTED;
+    // one medium throughout (every plane in the relaxing branch)
+    if (!(prm->z_min >= prm->water_tissue)) return BD_ERR_UNSUPPORTED;
+    if (prm->arr_nx <= 0 || prm->arr_ny <= 0 || !(prm->arr_pitch > 0)) return BD_ERR_INVALID;
+    hipStream_t st = as_stream(stream);
+    if (dtype == BD_F32)
+        return adjoint_t<float>(prm, Nxy, Nz, n_steps, checkpoint_every, elem_amp, elem_phase, obj, J, grad_amp,
+                                grad_phase, timings_ms, st);
+    return adjoint_t<double>(prm, Nxy, Nz, n_steps, checkpoint_every, elem_amp, elem_phase, obj, J, grad_amp,
+                             grad_phase, timings_ms, st);
+}

@@ -11,6 +11,7 @@
 // pressure, per-step energy and the focus sample are fused reductions.
 #include "common.h"
 #include "wv_march.h"
+#include "wv_internal.h"
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
@@ -325,6 +326,7 @@ static WvGeom make_geom(const bd_westervelt_params* p, int Nxy, int Nz, int vari
     g.sponge_w = p->sponge_cells;
     g.full_domain = p->full_domain;
     g.sponge_alpha = p->sponge_alpha;
+    g.obj_on = 0; g.ocx = g.ocy = g.ocz = 0; g.or2 = 0.0;
     const double dV = p->dxy * p->dxy * p->dz;
     g.escale[0] = dV / (p->rho * p->c_inf * p->c_inf);
     g.escale[1] = dV / (p->rho_tissue * p->c_inf_tissue * p->c_inf_tissue);
@@ -411,6 +413,46 @@ static void set_fused_z(WvMarchArgs<T>& a, const bd_westervelt_params* p, int on
     a.d_time = nullptr;
     a.t_next = 0.0;
     a.src = make_src(p, amp, phase);
+}
+
+template <typename T>
+static int forward_step_t(const bd_westervelt_params* prm, int Nxy, int Nz, const WvStepPtrs& q, int step,
+                          double t_next, const double* amp, const double* phase, double* epart,
+                          const WvObjective* obj, hipStream_t st) {
+    WvMarchArgs<T> a;
+    a.pn = (T*)q.pn; a.p = (const T*)q.p; a.pm1 = (const T*)q.pm1; a.pm2 = (const T*)q.pm2;
+    a.pm3 = (const T*)q.pm3; a.pm4 = (const T*)q.pm4;
+    a.r1 = (T*)q.r1; a.r1m1 = (const T*)q.r1m1; a.r1m2 = (const T*)q.r1m2;
+    a.r2 = (T*)q.r2; a.r2m1 = (const T*)q.r2m1; a.r2m2 = (const T*)q.r2m2;
+    a.pmax = nullptr; a.epart = epart; a.focus = nullptr; a.d_step = nullptr; a.focus_idx = -1;
+    int variant, blocks;
+    launch_shape(&variant, &blocks);
+    a.g = make_geom(prm, Nxy, Nz, variant, blocks);
+    if (obj) { a.g.obj_on = 1; a.g.ocx = obj->cx; a.g.ocy = obj->cy; a.g.ocz = obj->cz; a.g.or2 = obj->r2; }
+    a.c = make_coef<T>(prm);
+    set_fused_z<T>(a, prm, 1, amp, phase);
+    a.t_next = t_next;
+    int rc = launch_march<T>(a, variant, step, st);
+    if (rc) return rc;
+    const WvFaceArgs<T> f = make_faces<T>(prm, Nxy, Nz, (T*)q.pn, (const T*)q.p, nullptr);
+    return launch_xyfaces<T>(f, st);
+}
+
+int wv_forward_step(const bd_westervelt_params* prm, int dtype, int Nxy, int Nz, const WvStepPtrs& q, int step,
+                    double t_next, const double* amp, const double* phase, double* epart, const WvObjective* obj,
+                    hipStream_t st) {
+    if (dtype == BD_F32) return forward_step_t<float>(prm, Nxy, Nz, q, step, t_next, amp, phase, epart, obj, st);
+    return forward_step_t<double>(prm, Nxy, Nz, q, step, t_next, amp, phase, epart, obj, st);
+}
+
+int wv_forward_nblocks(const bd_westervelt_params* prm, int Nxy, int Nz) {
+    int variant, blocks;
+    launch_shape(&variant, &blocks);
+    return make_geom(prm, Nxy, Nz, variant, blocks).nblocks;
+}
+
+int wv_check_params(const bd_westervelt_params* p, int dtype, int Nxy, int Nz) {
+    return check_params(p, dtype, Nxy, Nz);
 }
 
 }  // namespace bd

@@ -32,6 +32,9 @@ struct WvGeom {
     int sponge_w, full_domain;
     double sponge_alpha;
     double escale[2];                  // dV / (rho c^2), water / tissue
+    // objective mode: the two partials are sum p^2 inside / outside a sphere
+    int obj_on, ocx, ocy, ocz;
+    double or2;
 };
 
 struct WvSrc {
@@ -351,9 +354,15 @@ wv_march(WvMarchArgs<T> a, int step_arg) {
                 if (a.pmax && pn > cur.pmx[r]) bstore<T>(pn, make_rsrc(a.pmax + zo, pbytes), o);
                 if (a.focus && (long long)(zo + cidx[r]) == a.focus_idx) a.focus[step] = (double)pn;
                 lastp[r] = pn;
-                const T dd = pn - pc;
-                e0.add(pn * pn * T(a.g.escale[reg]));
-                e1.add(dd * dd - pn * lap);
+                if (a.g.obj_on) {
+                    const int dx = ix - a.g.ocx, dy = iyr[r] - a.g.ocy, dz = z - a.g.ocz;
+                    if ((double)(dx * dx + dy * dy + dz * dz) <= a.g.or2) e0.add(pn * pn);
+                    else e1.add(pn * pn);
+                } else {
+                    const T dd = pn - pc;
+                    e0.add(pn * pn * T(a.g.escale[reg]));
+                    e1.add(dd * dd - pn * lap);
+                }
             }
         }
 #pragma unroll

@@ -289,6 +289,28 @@ size_t bd_gilmore_workspace_bytes(void);
 int bd_gilmore_integrate(const bd_gilmore_params* prm, const bd_gilmore_batch* batch,
                          void* workspace, void* stream);
 
+/* ---- adjoint: phased-array gradient (no reference counterpart) ---------- */
+
+/* Objective J = -w_target sum_{n=1..Nt} sum_{target} p^2 + w_outside sum_n sum_{outside} p^2
+ * over interior cells; the target is the sphere |cell - (cx,cy,cz)| <= radius (cells). */
+typedef struct bd_adjoint_objective {
+    int32_t cx, cy, cz, pad0;
+    double radius, w_target, w_outside;
+    double reserved[4];
+} bd_adjoint_objective;
+
+/* Runs n_steps forward steps of the full-domain phased-array model
+ * (full_domain=1, BD_BC_ZERO, BD_SRC_ARRAY, one medium) from rest, returns J
+ * (host) and, when grad_amp/grad_phase are non-NULL (device, arr_nx*arr_ny
+ * doubles each), dJ/dA_e and dJ/dphi_e from the discrete adjoint.  Forward
+ * levels are checkpointed every checkpoint_every steps and recomputed per
+ * segment (checkpoint_every >= n_steps keeps them all).  timings_ms (host,
+ * nullable) receives forward / recompute / backward wall times.  Synchronous. */
+int bd_westervelt_adjoint(const bd_westervelt_params* prm, int dtype, int Nxy, int Nz, int n_steps,
+                          int checkpoint_every, const double* elem_amp, const double* elem_phase,
+                          const bd_adjoint_objective* obj, double* J, double* grad_amp,
+                          double* grad_phase, double* timings_ms, void* stream);
+
 /* Library metadata */
 const char* bd_version(void);
 int bd_device_count(void);
