@@ -13,6 +13,8 @@ from __future__ import annotations
 
 import argparse
 import json
+
+import numpy as np
 import math
 import os
 import sys
@@ -32,6 +34,7 @@ def log(msg: str) -> None:
 def cpu_baseline(n: int, budget_s: float) -> dict:
     """FP64 oracle (serial C++ restatement) on the host cores, bounded sample."""
     import numpy as np
+
     import oracle
     from bubbledynamics_amd import presets
     orc = oracle.load()
@@ -69,6 +72,94 @@ def read_traffic(profile: Path) -> float | None:
     return None
 
 
+def _timed(fn):
+    import torch
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    out = fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return out, e0.elapsed_time(e1) / 1e3
+
+
+def gilmore_figures() -> dict:
+    """1e6 bubbles, adaptive DP5(4) (rtol 1e-8, atol 1e-12), 20 us, FP64 and the
+    FP32/FP64 split; 1,024 bubbles, RK4 dt = 0.1 ns, 20 us, FP64 with R(t) every 1 ns."""
+    import numpy as np
+    from bubbledynamics_amd import gilmore as gm
+    from bubbledynamics_amd._ffi import BD_GM_DP45, BD_GM_RK4, BD_GM_SPLIT
+    p = gm.workload_params()
+    R0, amp, ph, fr = gm.config3_inputs(1_000_000, 3)
+    b = gm.BubbleBatch(R0, amp, ph, fr)
+    gm.BubbleBatch(R0[:4096], amp[:4096], ph[:4096], fr).run(p, BD_GM_DP45, 2e-6, 1e-13)   # warm-up
+    o64, t64 = _timed(lambda: b.run(p, BD_GM_DP45, 20e-6, 1e-13, rtol=1e-8, atol=1e-12))
+    s64 = int(o64["n_acc"].sum().item())
+    rej64 = int(o64["n_rej"].sum().item())
+    o32, t32 = _timed(lambda: b.run(p, BD_GM_DP45, 20e-6, 1e-13, precision=BD_GM_SPLIT, rtol=1e-8,
+                                    atol=1e-12, split_ratio=0.2))
+    s32 = int(o32["n_acc"].sum().item())
+    rel = (o32["Rmax"] / o64["Rmax"] - 1).abs().cpu().numpy()
+    flips = float(((o32["t_collapse"] > 0) != (o64["t_collapse"] > 0)).float().mean().item())
+    R0c, ampc, phc, frc = gm.config0_inputs(1024, 0)
+    bc = gm.BubbleBatch(R0c, ampc, phc, frc)
+    oc, tc = _timed(lambda: bc.run(p, BD_GM_RK4, 20e-6, 1e-10, out_every=10, n_out=20001))
+    sc = int(oc["n_acc"].sum().item())
+    return {
+        "dp45_1e6_fp64": {"bubble_steps_per_s": s64 / t64, "accepted_steps": s64, "rejected": rej64,
+                          "seconds": t64, "collapsed_frac": float((o64["t_collapse"] > 0).float().mean().item())},
+        "dp45_1e6_split": {"bubble_steps_per_s": s32 / t32, "accepted_steps": s32, "seconds": t32,
+                           "rmax_rel_err_vs_fp64": {"median": float(np.median(rel)),
+                                                    "p90": float(np.quantile(rel, 0.9)),
+                                                    "p99": float(np.quantile(rel, 0.99)), "max": float(rel.max())},
+                           "collapse_flag_flips": flips},
+        "rk4_1024_fp64": {"bubble_steps_per_s": sc / tc, "steps": sc, "seconds": tc,
+                          "collapsed_frac": float((oc["t_collapse"] > 0).float().mean().item())},
+        "unit": "bubble-steps/s", "tableau": "Dormand-Prince 1980",
+    }
+
+
+def adjoint_figures() -> dict:
+    """128^3, 64-element array, 1,500 steps, FP32; checkpoints every 50 steps and
+    every level kept; cost as a multiple of the forward solve."""
+    import math
+    import torch
+    from bubbledynamics_amd import presets
+    from bubbledynamics_amd.adjoint import adjoint_gradient, objective
+    from bubbledynamics_amd.gilmore import seeded_uniforms
+    n, steps, ne = 128, 1500, 8
+    p = presets.phased_array(n, n_el=ne, dx=2e-4)
+    u = seeded_uniforms(4, 2 * ne * ne)
+    ph, amp = 2 * math.pi * u[:ne * ne], 0.5e6 + 0.5e6 * u[ne * ne:]
+    obj = objective(n, n, steps, (n // 2, n // 2, n // 2), 5.0)
+    adjoint_gradient(p, n, 60, amp, ph, objective(n, n, 60, (n // 2,) * 3, 5.0), 50, torch.float32)  # warm-up
+    out = {}
+    for K in (50, steps):
+        J, gA, gP, tm = adjoint_gradient(p, n, steps, amp, ph, obj, K, torch.float32)
+        out[f"checkpoint_{K}"] = {"forward_ms": tm[0], "recompute_ms": tm[1], "backward_ms": tm[2],
+                                  "cost_x_forward": (tm[0] + tm[1] + tm[2]) / tm[0], "J": J,
+                                  "grad_amp_norm": float(np.linalg.norm(gA))}
+    return out
+
+
+def coupled_figures() -> dict:
+    """256^3 CW Westervelt -> Q = 2 alpha I (5 Np/m/MHz) -> Pennes, 49-point spiral, 1 s dwell."""
+    import torch
+    from bubbledynamics_amd import workloads as wl
+    n = 256
+    Q, pw, tq = wl.heat_source(n)
+    run, secs, nsub = wl.pennes_spiral(Q, n)
+    cells = (n - 2) ** 3
+    rate = cells * nsub / secs
+    T = run.temperature()
+    return {"westervelt_steps": tq["steps"], "westervelt_s": tq["settle_s"] + tq["window_s"],
+            "q_max_w_m3": float(Q.max().item()), "bioheat_substeps": nsub, "bioheat_s": secs,
+            "bioheat_mcell_steps_per_s": rate / 1e6,
+            "bioheat_hbm_frac_28B": rate * 28 / PEAK_HBM,
+            "peak_rise_k": float((T.max() - run.p.T_art).item()),
+            "max_cem43_min": float(run.dose().max().item() / 60.0)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +169,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the Gilmore, adjoint and coupled bioheat figures")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first copy a buffer of this size 3 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -168,6 +261,17 @@ def main() -> int:
                               torch.linalg.vector_norm(run64.pmax)).item())}
         log(f"FP64 {extra['fp64']['value']} Mcell-steps/s; focal rel L2 FP32 vs FP64 = {rel:.2e}")
         del run64
+
+    if not args.no_extras:
+        for name, fn in (("gilmore", gilmore_figures), ("adjoint", adjoint_figures),
+                         ("coupled_bioheat", coupled_figures)):
+            try:
+                log(f"{name} ...")
+                extra[name] = fn()
+                log(f"{name}: {json.dumps(extra[name])}")
+            except Exception as exc:  # an auxiliary figure never blocks the main line
+                log(f"{name} failed: {exc!r}")
+                extra[name] = {"error": repr(exc)}
 
     cpu = None
     if not args.no_cpu:
