@@ -66,6 +66,7 @@ struct WvFaceArgs {
     T* P; const T* Pm1; T* pmax;
     int Nxy, Nz, lo, iz_tissue, boundary;
     double kz_w, kz_t, kxy_w, kxy_t;          // Courant numbers c dt / h (W:100, W:111, W:117-119)
+    int* tick; int tick_inc;                  // graph replay: advance the device step counter
 };
 
 // Z faces of the interior columns (W:101-113), innermost layer first; the
@@ -104,34 +105,57 @@ __global__ void __launch_bounds__(256) wv_zfaces(WvFaceArgs<T> a) {
 // Threads [0, nX) own one (iy, iz) row of the x layers (iy inside); threads
 // [nX, nX+nY) own one (ix, iz) column of the y layers (all ix).  A y-layer
 // thread in an x-layer column recomputes the x-layer value it starts from.
+// Each thread reads everything it needs (new inside values, p^n, old pmax)
+// before its first store, so the kernel costs one round of memory latency.
 template <typename T>
 __global__ void __launch_bounds__(256) wv_xyfaces(WvFaceArgs<T> a) {
     const int N = a.Nxy, Nz = a.Nz, lo = a.lo, nin = N - 3 - lo;
     const long long nX = (long long)nin * Nz, nY = (long long)N * Nz;
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    // the last face pass of a replayed graph advances the step counter that
+    // the next replay's interior kernels read (no kernel of this pass reads it)
+    if (a.tick && g == 0) *a.tick += a.tick_inc;
     if (g >= nX + nY) return;
     const size_t plane = (size_t)N * N;
     const bool zero = a.boundary == BD_BC_ZERO;
-    auto put = [&](size_t i, T v) { a.P[i] = v; if (a.pmax && v > a.pmax[i]) a.pmax[i] = v; };
+    const bool pm = a.pmax != nullptr;
+    // three layers outward from an inside cell (old pc, new v), old values m1..m3
+    auto chain = [&](T pc, T v, T m1, T m2, T m3, double k, T out[3]) {
+        out[0] = zero ? T(0) : mur<T>(pc, v, m1, k);
+        out[1] = zero ? T(0) : mur<T>(m1, out[0], m2, k);
+        out[2] = zero ? T(0) : mur<T>(m2, out[1], m3, k);
+    };
     if (g < nX) {
         const int iy = lo + (int)(g % nin), iz = (int)(g / nin);
         const double k = iz < a.iz_tissue ? a.kxy_w : a.kxy_t;
         const size_t r0 = (size_t)iz * plane + (size_t)iy * N;
-        {
-            const T b = a.P[r0 + N - 4];
-            const T m0 = a.Pm1[r0 + N - 4], m1 = a.Pm1[r0 + N - 3], m2 = a.Pm1[r0 + N - 2], m3 = a.Pm1[r0 + N - 1];
-            const T x0 = zero ? T(0) : mur<T>(m0, b, m1, k);
-            const T x1 = zero ? T(0) : mur<T>(m1, x0, m2, k);
-            const T x2 = zero ? T(0) : mur<T>(m2, x1, m3, k);
-            put(r0 + N - 3, x0); put(r0 + N - 2, x1); put(r0 + N - 1, x2);
+        T bh, mh[4], qh[3], bl = T(0), ml[4] = {T(0), T(0), T(0), T(0)}, ql[3] = {T(0), T(0), T(0)};
+        bh = a.P[r0 + N - 4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) mh[j] = a.Pm1[r0 + N - 4 + j];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) qh[j] = pm ? a.pmax[r0 + N - 3 + j] : T(0);
+        if (lo) {
+            bl = a.P[r0 + 3];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) ml[j] = a.Pm1[r0 + 3 - j];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) ql[j] = pm ? a.pmax[r0 + 2 - j] : T(0);
+        }
+        T xh[3], xl[3];
+        chain(mh[0], bh, mh[1], mh[2], mh[3], k, xh);
+        chain(ml[0], bl, ml[1], ml[2], ml[3], k, xl);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            a.P[r0 + N - 3 + j] = xh[j];
+            if (pm && xh[j] > qh[j]) a.pmax[r0 + N - 3 + j] = xh[j];
         }
         if (lo) {
-            const T b = a.P[r0 + 3];
-            const T m0 = a.Pm1[r0 + 3], m1 = a.Pm1[r0 + 2], m2 = a.Pm1[r0 + 1], m3 = a.Pm1[r0];
-            const T x0 = zero ? T(0) : mur<T>(m0, b, m1, k);
-            const T x1 = zero ? T(0) : mur<T>(m1, x0, m2, k);
-            const T x2 = zero ? T(0) : mur<T>(m2, x1, m3, k);
-            put(r0 + 2, x0); put(r0 + 1, x1); put(r0, x2);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                a.P[r0 + 2 - j] = xl[j];
+                if (pm && xl[j] > ql[j]) a.pmax[r0 + 2 - j] = xl[j];
+            }
         }
         return;
     }
@@ -153,25 +177,34 @@ __global__ void __launch_bounds__(256) wv_xyfaces(WvFaceArgs<T> a) {
         for (int x = 2; x >= ix; --x) v = mur<T>(a.Pm1[r0 + x + 1], v, a.Pm1[r0 + x], k);
         return v;
     };
-    {
-        const T b = after_x(N - 4);
-        const size_t c = zo + ix;
-        const T m0 = a.Pm1[c + (size_t)(N - 4) * N], m1 = a.Pm1[c + (size_t)(N - 3) * N];
-        const T m2 = a.Pm1[c + (size_t)(N - 2) * N], m3 = a.Pm1[c + (size_t)(N - 1) * N];
-        const T y0 = zero ? T(0) : mur<T>(m0, b, m1, k);
-        const T y1 = zero ? T(0) : mur<T>(m1, y0, m2, k);
-        const T y2 = zero ? T(0) : mur<T>(m2, y1, m3, k);
-        put(c + (size_t)(N - 3) * N, y0); put(c + (size_t)(N - 2) * N, y1); put(c + (size_t)(N - 1) * N, y2);
+    const size_t c = zo + ix;
+    T mh[4], qh[3], ml[4] = {T(0), T(0), T(0), T(0)}, ql[3] = {T(0), T(0), T(0)}, bl = T(0);
+    const T bh = after_x(N - 4);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) mh[j] = a.Pm1[c + (size_t)(N - 4 + j) * N];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) qh[j] = pm ? a.pmax[c + (size_t)(N - 3 + j) * N] : T(0);
+    if (lo) {
+        bl = after_x(3);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ml[j] = a.Pm1[c + (size_t)(3 - j) * N];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) ql[j] = pm ? a.pmax[c + (size_t)(2 - j) * N] : T(0);
+    }
+    T yh[3], yl[3];
+    chain(mh[0], bh, mh[1], mh[2], mh[3], k, yh);
+    chain(ml[0], bl, ml[1], ml[2], ml[3], k, yl);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        a.P[c + (size_t)(N - 3 + j) * N] = yh[j];
+        if (pm && yh[j] > qh[j]) a.pmax[c + (size_t)(N - 3 + j) * N] = yh[j];
     }
     if (lo) {
-        const T b = after_x(3);
-        const size_t c = zo + ix;
-        const T m0 = a.Pm1[c + 3 * (size_t)N], m1 = a.Pm1[c + 2 * (size_t)N];
-        const T m2 = a.Pm1[c + (size_t)N], m3 = a.Pm1[c];
-        const T y0 = zero ? T(0) : mur<T>(m0, b, m1, k);
-        const T y1 = zero ? T(0) : mur<T>(m1, y0, m2, k);
-        const T y2 = zero ? T(0) : mur<T>(m2, y1, m3, k);
-        put(c + 2 * (size_t)N, y0); put(c + (size_t)N, y1); put(c, y2);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            a.P[c + (size_t)(2 - j) * N] = yl[j];
+            if (pm && yl[j] > ql[j]) a.pmax[c + (size_t)(2 - j) * N] = yl[j];
+        }
     }
 }
 
@@ -184,7 +217,6 @@ __global__ void __launch_bounds__(256) wv_source_kernel(T* P, int Nxy, double t,
     P[(size_t)2 * Nxy * Nxy + g] = (T)source_value(src, ix, iy, t);
 }
 
-__global__ void tick_kernel(int* d_step, int inc) { if (threadIdx.x == 0 && blockIdx.x == 0) *d_step += inc; }
 __global__ void set_int_kernel(int* d, int v) { if (threadIdx.x == 0 && blockIdx.x == 0) *d = v; }
 
 // W:242-265
@@ -382,6 +414,8 @@ static WvFaceArgs<T> make_faces(const bd_westervelt_params* p, int Nxy, int Nz, 
     f.kz_t = p->c_inf_tissue * p->dt / p->dz;
     f.kxy_w = p->c_inf * p->dt / p->dxy;
     f.kxy_t = p->c_inf_tissue * p->dt / p->dxy;
+    f.tick = nullptr;
+    f.tick_inc = 0;
     return f;
 }
 
@@ -705,7 +739,8 @@ static int plan_step(bd_westervelt_plan* pl, int n, const int* d_step, int step_
     else a.t_next = pl->h_time[(size_t)n + 1];
     int rc = launch_march<T>(a, pl->variant, step_arg, st);
     if (rc) return rc;
-    const WvFaceArgs<T> f = make_faces<T>(&pl->prm, pl->Nxy, pl->Nz, P(n + 1), P(n), (T*)pl->pmax);
+    WvFaceArgs<T> f = make_faces<T>(&pl->prm, pl->Nxy, pl->Nz, P(n + 1), P(n), (T*)pl->pmax);
+    if (d_step && step_arg == 5) { f.tick = (int*)d_step; f.tick_inc = 6; }
     return launch_xyfaces<T>(f, st);
 }
 
@@ -726,10 +761,8 @@ static int plan_run_t(bd_westervelt_plan* pl, int n_steps, int use_graph, hipStr
             BD_HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
             BD_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
             int rc = BD_OK;
+            // six steps; the last face pass advances the device step counter
             for (int j = 0; j < 6 && rc == BD_OK; ++j) rc = plan_step<T>(pl, j, pl->d_step, j, cap);
-            if (rc == BD_OK) {
-                hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(1), 0, cap, pl->d_step, 6);
-            }
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(cap, &g);
             (void)hipStreamDestroy(cap);

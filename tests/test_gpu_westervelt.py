@@ -198,6 +198,39 @@ def test_phased_array_full_domain_parity(orc, dtype):
     assert np.linalg.norm(host(run.focus) - fs) / np.linalg.norm(fs) <= tol * 5
 
 
+@pytest.mark.parametrize("n,full", [(67, 0), (70, 1), (40, 1), (44, 0)])
+def test_plan_faces_at_tile_edges(orc, n, full):
+    # Mur faces of a run: quarter (67) and full (70) domains
+    # put the last inside column and row on the last lane / row of a 64 x 4
+    # tile, so the corner layers come from neighbouring tiles' halo; FP64
+    # state and pmax must match the oracle's face order everywhere.
+    nz, steps = 24, 90
+    p = presets.reference_scaled(n, nz)
+    p.d = 0.8 * (n - 4) * p.dxy
+    p.alpha, p.alpha_hole = math.asin(0.9), 0.0
+    if full:
+        p.full_domain = 1
+        p.xy_min = -(n - 1) / 2 * p.dxy
+    focus = (n // 2, n // 2, nz // 2)
+    st, pmax, en, fs = _oracle_run(orc, p, n, nz, steps, focus)
+    run = WesterveltRun(p, n, nz, dtype=torch.float64, max_steps=steps, focus=focus)
+    run.run(steps, use_graph=True)
+    run.finalize()
+    torch.cuda.synchronize()
+    ref = st[steps % 6]
+    got = host(run.pressure(0))
+    face = np.zeros((nz, n, n), bool)
+    face[:, :, n - 3:] = face[:, n - 3:, :] = True
+    if full:
+        face[:, :, :3] = face[:, :3, :] = True
+    assert np.abs(ref[face]).max() > 1e-3 * np.abs(ref).max()     # the faces are exercised
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() <= 1e-11 * scale
+    assert np.abs(host(run.pmax) - pmax).max() <= 1e-11 * np.abs(pmax).max()
+    e = host(run.energy)
+    assert np.abs(e[:, 0] - en[:, 0]).max() <= 1e-11 * np.abs(en[:, 0]).max()
+
+
 def test_tone_burst_small_fp32_vs_fp64(orc):
     # the benchmark workload's physics at 64^3: FP32 focal pressure against the
     # FP64 oracle, relative L2 over the whole record
