@@ -436,7 +436,7 @@ static int launch_zfaces(const WvFaceArgs<T>& f, hipStream_t st) {
     return BD_OK;
 }
 
-// Fills the march arguments that fuse the source plane and the z faces.
+// Fills the march arguments that fuse the source plane and the z, x and y faces.
 template <typename T>
 static void set_fused_z(WvMarchArgs<T>& a, const bd_westervelt_params* p, int on, const double* amp,
                         const double* phase) {
@@ -444,6 +444,9 @@ static void set_fused_z(WvMarchArgs<T>& a, const bd_westervelt_params* p, int on
     a.boundary = p->boundary;
     a.kz_w = p->c_inf * p->dt / p->dz;
     a.kz_t = p->c_inf_tissue * p->dt / p->dz;
+    a.fuse_xy = on;
+    a.kxy_w = p->c_inf * p->dt / p->dxy;
+    a.kxy_t = p->c_inf_tissue * p->dt / p->dxy;
     a.d_time = nullptr;
     a.t_next = 0.0;
     a.src = make_src(p, amp, phase);
@@ -458,7 +461,7 @@ static int forward_step_t(const bd_westervelt_params* prm, int Nxy, int Nz, cons
     a.pm3 = (const T*)q.pm3; a.pm4 = (const T*)q.pm4;
     a.r1 = (T*)q.r1; a.r1m1 = (const T*)q.r1m1; a.r1m2 = (const T*)q.r1m2;
     a.r2 = (T*)q.r2; a.r2m1 = (const T*)q.r2m1; a.r2m2 = (const T*)q.r2m2;
-    a.pmax = nullptr; a.epart = epart; a.focus = nullptr; a.d_step = nullptr; a.focus_idx = -1;
+    a.pmax = nullptr; a.epart = epart; a.focus = nullptr; a.d_step = nullptr; a.d_step_next = nullptr; a.focus_idx = -1;
     int variant, blocks;
     launch_shape(&variant, &blocks);
     a.g = make_geom(prm, Nxy, Nz, variant, blocks);
@@ -466,10 +469,7 @@ static int forward_step_t(const bd_westervelt_params* prm, int Nxy, int Nz, cons
     a.c = make_coef<T>(prm);
     set_fused_z<T>(a, prm, 1, amp, phase);
     a.t_next = t_next;
-    int rc = launch_march<T>(a, variant, step, st);
-    if (rc) return rc;
-    const WvFaceArgs<T> f = make_faces<T>(prm, Nxy, Nz, (T*)q.pn, (const T*)q.p, nullptr);
-    return launch_xyfaces<T>(f, st);
+    return launch_march<T>(a, variant, step, st);
 }
 
 int wv_forward_step(const bd_westervelt_params* prm, int dtype, int Nxy, int Nz, const WvStepPtrs& q, int step,
@@ -506,7 +506,7 @@ static int space6_t(const bd_westervelt_params* prm, void* Pnext, const void* P,
     a.pm3 = (const T*)Pm3; a.pm4 = (const T*)Pm4;
     a.r1 = (T*)R1; a.r1m1 = (const T*)R1m1; a.r1m2 = (const T*)R1m2;
     a.r2 = (T*)R2; a.r2m1 = (const T*)R2m1; a.r2m2 = (const T*)R2m2;
-    a.pmax = nullptr; a.epart = nullptr; a.focus = nullptr; a.d_step = nullptr; a.focus_idx = -1;
+    a.pmax = nullptr; a.epart = nullptr; a.focus = nullptr; a.d_step = nullptr; a.d_step_next = nullptr; a.focus_idx = -1;
     int variant, blocks;
     launch_shape(&variant, &blocks);
     a.g = make_geom(prm, Nxy, Nz, variant, blocks);
@@ -652,10 +652,10 @@ struct bd_westervelt_plan {
     int steps = 0;
     double* d_epart = nullptr;
     double* d_time = nullptr;
-    int* d_step = nullptr;
+    int* d_step = nullptr;              // two step counters, one per graph of the pair
     std::vector<double> h_time;
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
     int device = 0;
 };
 
@@ -686,7 +686,7 @@ extern "C" int bd_westervelt_plan_create(const bd_westervelt_params* prm, int dt
     if (hipGetDevice(&pl->device) != hipSuccess ||
         hipMalloc(&pl->d_epart, sizeof(double) * 2 * (size_t)pl->g.nblocks * max_steps) != hipSuccess ||
         hipMalloc(&pl->d_time, sizeof(double) * ((size_t)max_steps + 1)) != hipSuccess ||
-        hipMalloc(&pl->d_step, sizeof(int)) != hipSuccess) {
+        hipMalloc(&pl->d_step, 2 * sizeof(int)) != hipSuccess) {
         bd_westervelt_plan_destroy(pl);
         return BD_ERR_ALLOC;
     }
@@ -711,13 +711,16 @@ extern "C" int bd_westervelt_plan_bind(bd_westervelt_plan* pl, void* state, void
     pl->state = state; pl->pmax = pmax; pl->energy = energy; pl->focus = focus;
     pl->amp = amp; pl->phase = phase;
     pl->steps = 0;
-    if (pl->gexec) { (void)hipGraphExecDestroy(pl->gexec); pl->gexec = nullptr; }
-    if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+    for (int k = 0; k < 2; ++k) {
+        if (pl->gexec[k]) { (void)hipGraphExecDestroy(pl->gexec[k]); pl->gexec[k] = nullptr; }
+        if (pl->graph[k]) { (void)hipGraphDestroy(pl->graph[k]); pl->graph[k] = nullptr; }
+    }
     return BD_OK;
 }
 
 template <typename T>
-static int plan_step(bd_westervelt_plan* pl, int n, const int* d_step, int step_arg, hipStream_t st) {
+static int plan_step(bd_westervelt_plan* pl, int n, const int* d_step, int step_arg, hipStream_t st,
+                     int* d_step_next = nullptr) {
     const size_t N = (size_t)pl->Nxy * pl->Nxy * pl->Nz;
     T* S = (T*)pl->state;
     auto P = [&](int m) { return S + (size_t)(((m % 6) + 6) % 6) * N; };
@@ -731,17 +734,14 @@ static int plan_step(bd_westervelt_plan* pl, int n, const int* d_step, int step_
     a.epart = pl->energy ? pl->d_epart : nullptr;
     a.focus = pl->focus;
     a.d_step = d_step;
+    a.d_step_next = d_step_next;
     a.focus_idx = pl->focus_idx;
     a.g = pl->g;
     a.c = make_coef<T>(&pl->prm);
     set_fused_z<T>(a, &pl->prm, 1, pl->amp, pl->phase);
     if (d_step) a.d_time = pl->d_time;            // graph node: t = d_time[*d_step + j + 1]
     else a.t_next = pl->h_time[(size_t)n + 1];
-    int rc = launch_march<T>(a, pl->variant, step_arg, st);
-    if (rc) return rc;
-    WvFaceArgs<T> f = make_faces<T>(&pl->prm, pl->Nxy, pl->Nz, P(n + 1), P(n), (T*)pl->pmax);
-    if (d_step && step_arg == 5) { f.tick = (int*)d_step; f.tick_inc = 6; }
-    return launch_xyfaces<T>(f, st);
+    return launch_march<T>(a, pl->variant, step_arg, st);
 }
 
 template <typename T>
@@ -756,25 +756,29 @@ static int plan_run_t(bd_westervelt_plan* pl, int n_steps, int use_graph, hipStr
         if (!use_graph) continue;
     }
     if (use_graph && end - n >= 6) {
-        if (!pl->gexec) {
+        // A pair of 6-step graphs: graph k reads step counter k and its last
+        // kernel writes counter 1-k (= counter k + 6) for the next replay, so
+        // no kernel ever writes the counter it reads.
+        for (int k = 0; k < 2; ++k) {
+            if (pl->gexec[k]) continue;
             hipStream_t cap;
             BD_HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
             BD_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
             int rc = BD_OK;
-            // six steps; the last face pass advances the device step counter
-            for (int j = 0; j < 6 && rc == BD_OK; ++j) rc = plan_step<T>(pl, j, pl->d_step, j, cap);
+            for (int j = 0; j < 6 && rc == BD_OK; ++j)
+                rc = plan_step<T>(pl, j, pl->d_step + k, j, cap, j == 5 ? pl->d_step + (1 - k) : nullptr);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(cap, &g);
             (void)hipStreamDestroy(cap);
             if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
             if (e != hipSuccess) return BD_ERR_HIP;
-            pl->graph = g;
-            BD_HIP_CHECK(hipGraphInstantiate(&pl->gexec, g, nullptr, nullptr, 0));
+            pl->graph[k] = g;
+            BD_HIP_CHECK(hipGraphInstantiate(&pl->gexec[k], g, nullptr, nullptr, 0));
         }
         hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, st, pl->d_step, n);
         BD_LAUNCH_CHECK();
-        while (end - n >= 6) {
-            BD_HIP_CHECK(hipGraphLaunch(pl->gexec, st));
+        for (int k = 0; end - n >= 6; k ^= 1) {
+            BD_HIP_CHECK(hipGraphLaunch(pl->gexec[k], st));
             n += 6;
         }
     }
@@ -814,6 +818,7 @@ static int plan_interior_only_t(bd_westervelt_plan* pl, hipStream_t st) {
     a.epart = pl->energy ? pl->d_epart : nullptr;
     a.focus = pl->focus;
     a.d_step = nullptr;
+    a.d_step_next = nullptr;
     a.focus_idx = pl->focus_idx;
     a.g = pl->g;
     a.c = make_coef<T>(&pl->prm);
@@ -858,8 +863,10 @@ extern "C" int bd_westervelt_plan_slot(const bd_westervelt_plan* pl, int which, 
 
 extern "C" int bd_westervelt_plan_destroy(bd_westervelt_plan* pl) {
     if (!pl) return BD_OK;
-    if (pl->gexec) (void)hipGraphExecDestroy(pl->gexec);
-    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    for (int k = 0; k < 2; ++k) {
+        if (pl->gexec[k]) (void)hipGraphExecDestroy(pl->gexec[k]);
+        if (pl->graph[k]) (void)hipGraphDestroy(pl->graph[k]);
+    }
     if (pl->d_epart) (void)hipFree(pl->d_epart);
     if (pl->d_time) (void)hipFree(pl->d_time);
     if (pl->d_step) (void)hipFree(pl->d_step);

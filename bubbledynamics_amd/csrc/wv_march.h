@@ -9,7 +9,11 @@
 //     coalesced load of plane z+3;
 //   * p^{n-1..n-4}, R1/R2 history and pmax are streamed once;
 //   * with PF, all loads of plane z+1 are issued before plane z is computed.
-// Fused outputs: pmax, per-block energy partials, the focus sample.
+// Fused outputs: pmax, per-block energy partials, the focus sample; with
+// fuse_z the source plane and the z faces; with fuse_xy the x/y Mur faces,
+// which a workgroup at the edge of the grid forms after its march as
+// independent (plane, layer chain) tasks, every task's loads issued before
+// any store so that the pass costs about one round of memory latency.
 #pragma once
 #include "common.h"
 #include <type_traits>
@@ -92,12 +96,15 @@ struct WvMarchArgs {
     double* epart;         // [step][nblocks][2], nullable
     double* focus;         // [step], nullable
     const int* d_step;     // step base for graph replay, nullable
+    int* d_step_next;      // written with the next replay's base (last kernel of a graph), nullable
     long long focus_idx;
     WvGeom g;
     WvCoef<T> c;
     // source plane and z faces fused into the first / last z-chunk
     int fuse_z, boundary;
     double kz_w, kz_t;         // Courant numbers of the z faces (W:100, W:111)
+    int fuse_xy;               // x/y faces after the march (requires fuse_z)
+    double kxy_w, kxy_t;       // Courant numbers of the x/y faces (W:117-119)
     const double* d_time;      // time table (graph replay), t = d_time[step+1]
     double t_next;             // t^{n+1} when d_time is null
     WvSrc src;
@@ -138,6 +145,7 @@ wv_march(WvMarchArgs<T> a, int step_arg) {
     __shared__ double red[NT / kWave];
 
     const int step = a.d_step ? (*a.d_step + step_arg) : step_arg;
+    if (a.d_step_next && blockIdx.x == 0 && threadIdx.x == 0) *a.d_step_next = step + 1;
     const int lb = xcd_remap((int)blockIdx.x, a.g.nblocks);
     const int zc = lb % a.g.nzc, t2 = lb / a.g.nzc;
     const int bx = t2 % a.g.ntx, by = t2 / a.g.ntx;
@@ -397,6 +405,132 @@ wv_march(WvMarchArgs<T> a, int step_arg) {
                 if (u0 > a.pmax[c + plane]) a.pmax[c + plane] = u0;
                 if (u1 > a.pmax[c + 2 * plane]) a.pmax[c + 2 * plane] = u1;
                 if (u2 > a.pmax[c + 3 * plane]) a.pmax[c + 3 * plane] = u2;
+            }
+        }
+    }
+    // Fused x/y faces (W:88-135, the order of wv_xyfaces): x layers of the
+    // inside rows, y layers of the inside columns, then the corners from the
+    // x layers of the owner row, for every plane this workgroup wrote (its
+    // march planes and, for the first / last z-chunk, the z-face planes).
+    // The owners' new values were written by this workgroup before the barrier.
+    if (a.fuse_xy) {
+        __syncthreads();
+        const bool zero = a.boundary == BD_BC_ZERO;
+        const int lo = a.g.xlo;
+        const int cx0 = a.g.xlo + bx * TX, cx1 = min(cx0 + TX, N);
+        const int ry1 = min(ybase + ROWS, N);
+        const int oxh = (N - 4 >= cx0 && N - 4 < cx1) ? 1 : 0;
+        const int oxl = (lo && 3 >= cx0 && 3 < cx1) ? 1 : 0;
+        const int oyh = (N - 4 >= ybase && N - 4 < ry1) ? 1 : 0;
+        const int oyl = (lo && 3 >= ybase && 3 < ry1) ? 1 : 0;
+        const int ra = max(ybase, lo), nr = max(0, min(ry1, N - 3) - ra);   // inside rows of the tile
+        const int ca = max(cx0, lo), nc = max(0, min(cx1, N - 3) - ca);     // inside columns
+        const int nxs = oxh + oxl, nys = oyh + oyl;
+        const int nch = nxs * nr + nys * nc;             // chains of one plane
+        const int ncor = nxs * nys;                      // corners of one plane
+        const int nm = z1 - z0;
+        const int nfirst = (a.fuse_z && zc == 0) ? 3 : 0;
+        const int nlast = (a.fuse_z && zc == a.g.nzc - 1) ? 3 : 0;
+        const int npl = nm + nfirst + nlast;
+        auto plane_of = [&](int i) { return i < nm ? z0 + i : (i - nm < nfirst ? i - nm : a.g.Nz - 3 + (i - nm - nfirst)); };
+        const T* __restrict__ P0 = a.p;
+        T* Pn = a.pn;
+        T* PM = a.pmax;
+        const bool pm = PM != nullptr;
+        // chains, KB per thread per batch: all loads of the batch, then the stores
+        constexpr int KB = sizeof(T) == 4 ? 8 : 4;
+        const int tch = npl * nch;
+        for (int t0 = 0; t0 < tch; t0 += KB * NT) {
+            int zz[KB], off[KB], d[KB];       // plane, in-plane offset of the owner, layer stride
+            T v[KB], m[KB][4], q[KB][3];
+#pragma unroll
+            for (int j = 0; j < KB; ++j) {
+                const int t = t0 + j * NT + (int)threadIdx.x;
+                zz[j] = -1; off[j] = 0; d[j] = 0;
+                v[j] = T(0);
+                m[j][0] = m[j][1] = m[j][2] = m[j][3] = T(0);
+                q[j][0] = q[j][1] = q[j][2] = T(0);
+                if (t < tch) {
+                    zz[j] = plane_of(t / nch);
+                    int u = t % nch, ox, oy;
+                    if (u < nxs * nr) {
+                        const bool hi = oxh && u / nr == 0;
+                        ox = hi ? N - 4 : 3; oy = ra + u % nr; d[j] = hi ? 1 : -1;
+                    } else {
+                        u -= nxs * nr;
+                        const bool hi = oyh && u / nc == 0;
+                        ox = ca + u % nc; oy = hi ? N - 4 : 3; d[j] = hi ? N : -N;
+                    }
+                    off[j] = oy * N + ox;
+                    const T* p0 = P0 + (size_t)zz[j] * plane + off[j];
+                    v[j] = Pn[(size_t)zz[j] * plane + off[j]];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) m[j][k] = p0[k * d[j]];
+                    if (pm) {
+                        const T* q0 = PM + (size_t)zz[j] * plane + off[j];
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) q[j][k] = q0[(k + 1) * d[j]];
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < KB; ++j) {
+                if (zz[j] < 0) continue;
+                const double kf = zz[j] < a.g.iz_tissue ? a.kxy_w : a.kxy_t;
+                T w[3] = {T(0), T(0), T(0)};
+                if (!zero) {
+                    w[0] = mur<T>(m[j][0], v[j], m[j][1], kf);
+                    w[1] = mur<T>(m[j][1], w[0], m[j][2], kf);
+                    w[2] = mur<T>(m[j][2], w[1], m[j][3], kf);
+                }
+                T* o = Pn + (size_t)zz[j] * plane + off[j];
+                T* om = pm ? PM + (size_t)zz[j] * plane + off[j] : nullptr;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    o[(k + 1) * d[j]] = w[k];
+                    if (pm && w[k] > q[j][k]) om[(k + 1) * d[j]] = w[k];
+                }
+            }
+        }
+        // corners: the owner row's x layers (as its chain forms them), then
+        // the y layers of the three x-face columns
+        for (int t = threadIdx.x; t < npl * ncor; t += NT) {
+            const int z = plane_of(t / ncor), u = t % ncor;
+            const bool xhi = oxh && u / nys == 0, yhi = oyh && u % nys == 0;
+            const int sx = xhi ? 1 : -1;
+            const long long sy = yhi ? N : -N;
+            const long long o = (long long)z * (long long)plane + (long long)(yhi ? N - 4 : 3) * N + (xhi ? N - 4 : 3);
+            const double kf = z < a.g.iz_tissue ? a.kxy_w : a.kxy_t;
+            T c[4][4], old[3][3];
+            const T vo = Pn[o];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) c[j][k] = (j == 0 && k > 0) ? T(0) : P0[o + j * sx + k * sy];
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) old[j][k] = pm ? PM[o + (j + 1) * sx + (k + 1) * sy] : T(0);
+            T xv[3] = {T(0), T(0), T(0)};
+            if (!zero) {
+                xv[0] = mur<T>(c[0][0], vo, c[1][0], kf);
+                xv[1] = mur<T>(c[1][0], xv[0], c[2][0], kf);
+                xv[2] = mur<T>(c[2][0], xv[1], c[3][0], kf);
+            }
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                T w[3] = {T(0), T(0), T(0)};
+                if (!zero) {
+                    w[0] = mur<T>(c[j + 1][0], xv[j], c[j + 1][1], kf);
+                    w[1] = mur<T>(c[j + 1][1], w[0], c[j + 1][2], kf);
+                    w[2] = mur<T>(c[j + 1][2], w[1], c[j + 1][3], kf);
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const long long i = o + (j + 1) * sx + (k + 1) * sy;
+                    Pn[i] = w[k];
+                    if (pm && w[k] > old[j][k]) PM[i] = w[k];
+                }
             }
         }
     }
