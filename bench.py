@@ -3,8 +3,9 @@
 
 A step is one Westervelt time step over the 192^3 grid (interior 6th-order
 update with relaxation and nonlinearity, source plane, Mur faces, sponge,
-peak pressure, per-step energy and the focus sample), replayed from a
-captured 6-step hipGraph.  Inputs live in HBM before timing starts.
+peak pressure, per-step energy and the focus sample) -- one kernel launch,
+replayed from a pair of captured 6-step hipGraphs.  Inputs live in HBM
+before timing starts.
 
 Prints one JSON line.  `--gpus N` with N > 1 exits: this build measures one
 GPU (the z-slab sharding across GPUs is described in the design notes).
@@ -61,7 +62,7 @@ def cpu_baseline(n: int, budget_s: float) -> dict:
 
 
 def read_traffic(profile: Path) -> float | None:
-    """HBM bytes per launch of the interior kernel from the committed counter
+    """HBM bytes per launch of the step kernel from the committed counter
     summary (line 'hbm_bytes_per_launch: N'), or None."""
     try:
         for line in profile.read_text().splitlines():
@@ -220,26 +221,28 @@ def main() -> int:
     run32.finalize()
     torch.cuda.synchronize()
     pmax32 = run32.pmax.clone()
-    # interior kernel alone, HIP events on the stream it is launched on (these
-    # launches recompute the next step's interior and touch pmax, hence the copy)
+    # the step kernel alone, HIP events on the stream it is launched on (these
+    # launches recompute the next step and touch pmax, hence the copy)
     R = 50
     lib = _ffi.load()
     stream = torch.cuda.current_stream().cuda_stream
     import ctypes as C
     for _ in range(5):
-        _ffi.check("interior_only", lib.bd_westervelt_plan_interior_only(run32._plan, C.c_void_p(stream)))
+        _ffi.check("rerun_step", lib.bd_westervelt_plan_rerun_step(run32._plan, C.c_void_p(stream)))
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(R):
-        _ffi.check("interior_only", lib.bd_westervelt_plan_interior_only(run32._plan, C.c_void_p(stream)))
+        _ffi.check("rerun_step", lib.bd_westervelt_plan_rerun_step(run32._plan, C.c_void_p(stream)))
     e1.record()
     torch.cuda.synchronize()
     k_ms = e0.elapsed_time(e1) / R
     cells = int(lib.bd_westervelt_plan_interior_cells(run32._plan))
-    bytes_per_cell = 14 * 4          # 9 reads + 3 writes + pmax read/modify/write, FP32
-    alg_bytes = cells * bytes_per_cell
+    edge_cells = n ** 3 - cells
+    # inside cell: 9 reads + 3 writes + pmax read/modify/write; face / source
+    # cell: read p^n, write p^{n+1}, pmax read/modify/write (FP32)
+    alg_bytes = cells * 14 * 4 + edge_cells * 4 * 4
     achieved = alg_bytes / (k_ms / 1e3)
-    log(f"interior kernel {k_ms * 1e3:.2f} us -> {achieved / 1e9:.0f} GB/s algorithmic")
+    log(f"step kernel {k_ms * 1e3:.2f} us -> {achieved / 1e9:.0f} GB/s algorithmic")
 
     f32 = run32.focus.double().cpu().numpy()[:total]
     energy = run32.energy.cpu().numpy()[:total]
@@ -297,12 +300,13 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"westervelt_tone_burst_{n}", "grid": [n, n, n], "dx_m": p.dxy,
                    "dt_s": p.dt, "relaxation": 2, "source": "cap F#1, 1 MHz, 8 cycles, 1 MPa",
-                   "boundary": "Mur + 20-cell sponge", "hipgraph": "6-step"},
+                   "boundary": "Mur + 20-cell sponge", "hipgraph": "2 x 6-step"},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4),
                      "traffic": traffic,
                      "kernel": "wv_march<float,64,4,1,0>", "kernel_us": round(k_ms * 1e3, 2),
-                     "alg_bytes_per_launch": alg_bytes, "cells_per_launch": cells},
+                     "alg_bytes_per_launch": alg_bytes, "cells_per_launch": n ** 3,
+                     "inside_cells_per_launch": cells},
         "cpu_baseline": cpu,
         "focus": {"peak_pa": float(np.abs(f32).max()), "harmonics_pa": [float(h) for h in harm],
                   "energy_max_j": float(energy[:, 0].max())},

@@ -137,7 +137,7 @@ SIGNATURES: dict[str, tuple] = {
     "bd_westervelt_plan_steps": (_I, [_VP]),
     "bd_westervelt_plan_slot": (_I, [_VP, _I, _I]),
     "bd_westervelt_plan_destroy": (_I, [_VP]),
-    "bd_westervelt_plan_interior_only": (_I, [_VP, _VP]),
+    "bd_westervelt_plan_rerun_step": (_I, [_VP, _VP]),
     "bd_westervelt_plan_interior_cells": (C.c_longlong, [_VP]),
     "bd_westervelt_adjoint": (_I, [_PP, _I, _I, _I, _I, _I, _VP, _VP, C.POINTER(AdjointObjective),
                                    C.POINTER(_D), _VP, _VP, C.POINTER(_D), _VP]),

@@ -803,7 +803,7 @@ extern "C" int bd_westervelt_plan_run(bd_westervelt_plan* pl, int n_steps, int u
 }
 
 template <typename T>
-static int plan_interior_only_t(bd_westervelt_plan* pl, hipStream_t st) {
+static int plan_rerun_step_t(bd_westervelt_plan* pl, hipStream_t st) {
     const int n = pl->steps;
     const size_t N = (size_t)pl->Nxy * pl->Nxy * pl->Nz;
     T* S = (T*)pl->state;
@@ -827,10 +827,10 @@ static int plan_interior_only_t(bd_westervelt_plan* pl, hipStream_t st) {
     return launch_march<T>(a, pl->variant, n, st);
 }
 
-extern "C" int bd_westervelt_plan_interior_only(bd_westervelt_plan* pl, void* stream) {
+extern "C" int bd_westervelt_plan_rerun_step(bd_westervelt_plan* pl, void* stream) {
     if (!pl || !pl->state || pl->steps >= pl->max_steps) return BD_ERR_INVALID;
-    if (pl->dtype == BD_F32) return plan_interior_only_t<float>(pl, as_stream(stream));
-    return plan_interior_only_t<double>(pl, as_stream(stream));
+    if (pl->dtype == BD_F32) return plan_rerun_step_t<float>(pl, as_stream(stream));
+    return plan_rerun_step_t<double>(pl, as_stream(stream));
 }
 
 extern "C" long long bd_westervelt_plan_interior_cells(const bd_westervelt_plan* pl) {

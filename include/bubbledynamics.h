@@ -164,11 +164,12 @@ int bd_westervelt_plan_finalize(bd_westervelt_plan* plan, void* stream);
 int bd_westervelt_plan_steps(const bd_westervelt_plan* plan);
 int bd_westervelt_plan_slot(const bd_westervelt_plan* plan, int which, int level);
 int bd_westervelt_plan_destroy(bd_westervelt_plan* plan);
-/* Launches only the interior kernel of the next step (same arguments as the
- * real step, state not advanced) -- used to time that kernel in isolation.
- * Requires steps < max_steps. */
-int bd_westervelt_plan_interior_only(bd_westervelt_plan* plan, void* stream);
-/* Number of cells the interior kernel updates per launch. */
+/* Launches the step kernel of the next step again (same arguments as the
+ * real step, state not advanced; it rewrites the next levels and may raise
+ * pmax) -- used to time that kernel in isolation.  Requires steps < max_steps. */
+int bd_westervelt_plan_rerun_step(bd_westervelt_plan* plan, void* stream);
+/* Number of inside cells (the 6th-order update) per step; the other cells of
+ * the grid are Mur faces and the source plane. */
 long long bd_westervelt_plan_interior_cells(const bd_westervelt_plan* plan);
 
 /* ======================================================================== */
