@@ -36,7 +36,7 @@ def objective(n: int, nz: int, n_steps: int, center, radius: float, outside_weig
 
 def adjoint_gradient(p: WesterveltParams, n: int, n_steps: int, amp, phase, obj: AdjointObjective,
                      checkpoint_every: int = 50, dtype=torch.float64, gradient: bool = True):
-    """Returns (J, dJ/dA, dJ/dphi, timings_ms[forward, recompute, backward])."""
+    """Returns (J, dJ/dA, dJ/dphi, timings_ms[forward, recompute outside the sweep, adjoint sweep])."""
     amp_d = torch.as_tensor(np.asarray(amp, dtype=np.float64), device="cuda").contiguous()
     ph_d = torch.as_tensor(np.asarray(phase, dtype=np.float64), device="cuda").contiguous()
     ne = amp_d.numel()

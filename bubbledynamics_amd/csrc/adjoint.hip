@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -58,9 +59,24 @@ __device__ __forceinline__ T sponge_g(int ix, int iy, int iz, int N, int Nz, int
     return exp(-sp);
 }
 
-constexpr int kAX = 64, kAY = 4;
+// Tile shapes of the backward kernel (TX x TY threads, one column each).
+struct AdjTile { int ax, ay; };
+constexpr AdjTile kAdjTiles[] = {{64, 4}, {64, 8}, {32, 8}, {128, 4}, {128, 8}};
+constexpr int kAdjDefaultTile = 0, kAdjWideTile = 3;
 
+// Loads of one z plane for one thread (issued a plane ahead of their use).
 template <typename T>
+struct AdjPlane {
+    T wq;                       // w at (ix, iy, z+3): the z queue
+    T hx, hyb, hyt;             // x halo and bottom / top y halo of plane z (edge threads only)
+    T p0, p1, p2, p3;           // p^n .. p^{n-3}
+    T c0, c1, c2, c3;           // acc[n] .. acc[n-3]
+    T r1a, r1b, r2a, r2b;       // rho_v^{n+1}, rho_v^{n+2}
+};
+
+// The adjoint field is stored as w = G lam (zero outside the interior), so
+// the sponge factor is formed once per cell when lam^n is written.
+template <typename T, int kAX, int kAY>
 __global__ void __launch_bounds__(kAX * kAY) adj_step_kernel(AdjArgs<T> a) {
     __shared__ T s[2][kAY + 6][kAX + 6];
     const int lb = xcd_remap((int)blockIdx.x, a.nblocks);
@@ -70,35 +86,60 @@ __global__ void __launch_bounds__(kAX * kAY) adj_step_kernel(AdjArgs<T> a) {
     const int N = a.N, Nz = a.Nz;
     const size_t plane = (size_t)N * N;
     const int ix = 3 + bx * kAX + tx, iy = 3 + by * kAY + ty;
+    const bool inr = ix < N && iy < N;
     const bool act = ix <= N - 4 && iy <= N - 4;
     const int z0 = 3 + zc * a.zchunk;
     const int z1 = min(z0 + a.zchunk, Nz - 3);
-    // w = G lam^{n+1} on interior cells, 0 elsewhere
-    auto wval = [&](int x, int y, int z) -> T {
-        if (x < 3 || x > N - 4 || y < 3 || y > N - 4 || z < 3 || z > Nz - 4) return T(0);
-        return sponge_g<T>(x, y, z, N, Nz, a.sponge_w, a.sponge_alpha) * a.lam1[(size_t)z * plane + (size_t)y * N + x];
+    constexpr uint32_t ES = sizeof(T);
+    const uint32_t pbytes = (uint32_t)(plane * ES);
+    const uint32_t o = (uint32_t)(iy * N + ix) * ES;
+    // halo sources: x halo left (tx < 3) / right (tx >= kAX-3), y halo bottom / top
+    const bool xL = tx < 3, xR = tx >= kAX - 3, yB = ty < 3, yT = ty >= kAY - 3;
+    const int hxo = xL ? -3 : 3;                       // x offset of this thread's halo cell
+    // Halo cells beyond the grid are read only by threads outside the
+    // interior; their loads stay inside the field or return zero.
+    auto load_plane = [&](int z, AdjPlane<T>& v) {
+        const size_t zo = (size_t)z * plane;
+        const rsrc_t rw = make_rsrc(a.lam1 + zo, (size_t)(Nz - z) * plane * ES);
+        v.wq = inr ? bload<T>(rw, o, 3 * pbytes) : T(0);
+        v.hx = (xL || xR) && iy < N ? bload<T>(rw, (uint32_t)((int)o + hxo * (int)ES)) : T(0);
+        v.hyb = yB && ix < N ? bload<T>(rw, (uint32_t)((int)o - 3 * N * (int)ES)) : T(0);
+        v.hyt = yT && ix < N ? bload<T>(rw, (uint32_t)((int)o + 3 * N * (int)ES)) : T(0);
+        if (act) {
+            v.p0 = bload<T>(make_rsrc(a.p0 + zo, pbytes), o);
+            v.p1 = bload<T>(make_rsrc(a.p1 + zo, pbytes), o);
+            v.p2 = bload<T>(make_rsrc(a.p2 + zo, pbytes), o);
+            v.p3 = bload<T>(make_rsrc(a.p3 + zo, pbytes), o);
+            v.c0 = bload<T>(make_rsrc(a.acc0 + zo, pbytes), o);
+            v.c1 = bload<T>(make_rsrc(a.acc1 + zo, pbytes), o);
+            v.c2 = bload<T>(make_rsrc(a.acc2 + zo, pbytes), o);
+            v.c3 = bload<T>(make_rsrc(a.acc3 + zo, pbytes), o);
+            v.r1a = bload<T>(make_rsrc(a.r1a + zo, pbytes), o);
+            v.r1b = bload<T>(make_rsrc(a.r1b + zo, pbytes), o);
+            v.r2a = bload<T>(make_rsrc(a.r2a + zo, pbytes), o);
+            v.r2b = bload<T>(make_rsrc(a.r2b + zo, pbytes), o);
+        }
     };
     T q[7];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) q[k] = (ix < N && iy < N) ? wval(ix, iy, z0 - 3 + k) : T(0);
+    for (int k = 0; k < 6; ++k) q[k] = inr ? a.lam1[(size_t)(z0 - 3 + k) * plane + (size_t)iy * N + ix] : T(0);
     q[6] = T(0);
     if (zc == 0 && act && a.lsrc) {
         // source plane iz=2: only the z stencil reaches interior cells
         a.lsrc[(size_t)iy * N + ix] = a.kz * (T(270.0) * q[3] - T(27.0) * q[4] + T(2.0) * q[5]) / T(180.0);
     }
-    const int hx = 3 + bx * kAX - 3 + tx;   // left halo column of this thread (tx < 3)
-    const int hy = 3 + by * kAY - 3 + ty;   // bottom halo row (ty < 3)
+    AdjPlane<T> cur;
+    load_plane(z0, cur);
     for (int z = z0; z < z1; ++z) {
         const int b = (z - z0) & 1;
-        q[6] = (ix < N && iy < N) ? wval(ix, iy, z + 3) : T(0);
+        q[6] = cur.wq;
         s[b][ty + 3][tx + 3] = q[3];
-        if (tx < 3) s[b][ty + 3][tx] = (iy < N) ? wval(hx, iy, z) : T(0);
-        if (tx >= kAX - 3) s[b][ty + 3][tx + 6] = (iy < N && ix + 3 < N) ? wval(ix + 3, iy, z) : T(0);
-        if (ty < 3) s[b][ty][tx + 3] = (ix < N) ? wval(ix, hy, z) : T(0);
-        if (ty >= kAY - 3) s[b][ty + 6][tx + 3] = (ix < N && iy + 3 < N) ? wval(ix, iy + 3, z) : T(0);
+        if (xL || xR) s[b][ty + 3][tx + 3 + hxo] = cur.hx;
+        if (yB) s[b][ty][tx + 3] = cur.hyb;
+        if (yT) s[b][ty + 6][tx + 3] = cur.hyt;
         __syncthreads();
         if (act) {
-            const size_t i = (size_t)z * plane + (size_t)iy * N + ix;
+            const size_t zo = (size_t)z * plane;
             const T w = q[3];
             const T* row = &s[b][ty + 3][tx + 3];
             const T Lx = (T(-490.0) * w + T(270.0) * (row[1] + row[-1]) - T(27.0) * (row[2] + row[-2]) +
@@ -109,33 +150,37 @@ __global__ void __launch_bounds__(kAX * kAY) adj_step_kernel(AdjArgs<T> a) {
             const T Lz = (T(-490.0) * w + T(270.0) * (q[4] + q[2]) - T(27.0) * (q[5] + q[1]) +
                           T(2.0) * (q[6] + q[0])) / T(180.0);
             const T ST = a.kxy * (Lx + Ly) + a.kz * Lz;
-            const T pn = a.p0[i], pm1 = a.p1[i], pm2 = a.p2[i], pm3 = a.p3[i];
+            const T pn = cur.p0, pm1 = cur.p1, pm2 = cur.p2, pm3 = cur.p3;
             const T A = T(3.0) * pn - T(4.0) * pm1 + pm2;
             const T B = T(2.0) * pn - T(5.0) * pm1 + T(4.0) * pm2 - pm3;
-            const T rho1 = a.s * w + T(4.0) * a.b1 * a.r1a[i] - a.b1 * a.r1b[i];
-            const T rho2 = a.s * w + T(4.0) * a.b2 * a.r2a[i] - a.b2 * a.r2b[i];
+            const T rho1 = a.s * w + T(4.0) * a.b1 * cur.r1a - a.b1 * cur.r1b;
+            const T rho2 = a.s * w + T(4.0) * a.b2 * cur.r2a - a.b2 * cur.r2b;
             const T d = a.dlt * w + a.a1 * rho1 + a.a2 * rho2;
             const T e = a.bet * w;
             const int dx = ix - a.ocx, dy = iy - a.ocy, dz = z - a.ocz;
             const bool tgt = (double)(dx * dx + dy * dy + dz * dz) <= a.or2;
             const T dj = tgt ? T(-2.0 * a.wT) * pn : T(2.0 * a.wO) * pn;
-            a.lam0[i] = a.acc0[i] + T(2.0) * w + T(5.0) * d + e * (T(1.5) * A + B + T(2.0) * pn) + ST + dj;
-            a.acc1[i] = a.acc1[i] + (-w - T(18.0) * d + e * (T(-2.0) * A - T(5.0) * pn));
-            a.acc2[i] = a.acc2[i] + (T(24.0) * d + e * (T(0.5) * A + T(4.0) * pn));
-            a.acc3[i] = a.acc3[i] + (T(-14.0) * d - e * pn);
-            a.acc4[i] = T(3.0) * d;
-            a.r1o[i] = rho1;
-            a.r2o[i] = rho2;
+            const T lam = cur.c0 + T(2.0) * w + T(5.0) * d + e * (T(1.5) * A + B + T(2.0) * pn) + ST + dj;
+            const T g = sponge_g<T>(ix, iy, z, N, Nz, a.sponge_w, a.sponge_alpha);
+            bstore<T>(g * lam, make_rsrc(a.lam0 + zo, pbytes), o);
+            bstore<T>(cur.c1 + (-w - T(18.0) * d + e * (T(-2.0) * A - T(5.0) * pn)), make_rsrc(a.acc1 + zo, pbytes), o);
+            bstore<T>(cur.c2 + (T(24.0) * d + e * (T(0.5) * A + T(4.0) * pn)), make_rsrc(a.acc2 + zo, pbytes), o);
+            bstore<T>(cur.c3 + (T(-14.0) * d - e * pn), make_rsrc(a.acc3 + zo, pbytes), o);
+            bstore<T>(T(3.0) * d, make_rsrc(a.acc4 + zo, pbytes), o);
+            bstore<T>(rho1, make_rsrc(a.r1o + zo, pbytes), o);
+            bstore<T>(rho2, make_rsrc(a.r2o + zo, pbytes), o);
         }
 #pragma unroll
         for (int k = 0; k < 6; ++k) q[k] = q[k + 1];
+        if (z + 1 < z1) load_plane(z + 1, cur);
     }
 }
 
-// lam^{Nt} = dJ/dp^{Nt} on interior cells (nothing follows the last level).
+// lam^{Nt} = dJ/dp^{Nt} on interior cells (nothing follows the last level),
+// stored as w = G lam.
 template <typename T>
 __global__ void adj_init_kernel(T* lam, const T* p, int N, int Nz, int ocx, int ocy, int ocz, double or2, double wT,
-                                double wO) {
+                                double wO, int W, double alpha) {
     const size_t n = (size_t)N * N * Nz;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -145,6 +190,7 @@ __global__ void adj_init_kernel(T* lam, const T* p, int N, int Nz, int ocx, int 
         const int dx = ix - ocx, dy = iy - ocy, dz = iz - ocz;
         const bool tgt = (double)(dx * dx + dy * dy + dz * dz) <= or2;
         v = tgt ? T(-2.0 * wT) * p[i] : T(2.0 * wO) * p[i];
+        v = sponge_g<T>(ix, iy, iz, N, Nz, W, alpha) * v;
     }
     lam[i] = v;
 }
@@ -207,14 +253,17 @@ static int adjoint_t(const bd_westervelt_params* prm, int Nxy, int Nz, int Nt, i
     const int nseg = (Nt + K - 1) / K;
     const int nb = wv_forward_nblocks(prm, Nxy, Nz);
     // forward workspace: segment store p^{a-4..a+K}, R rings, checkpoints
-    DevBuf seg, rring, ckp, epart, jsum;
-    if (!seg.alloc(FB * (size_t)(K + 5)) || !rring.alloc(FB * 6) || !epart.alloc(sizeof(double) * 2 * nb * (size_t)Nt) ||
+    DevBuf seg[2], rring, ckp, epart, jsum;
+    if (!seg[0].alloc(FB * (size_t)(K + 5)) || !rring.alloc(FB * 6) || !epart.alloc(sizeof(double) * 2 * nb * (size_t)Nt) ||
         !jsum.alloc(sizeof(double) * 2))
         return BD_ERR_ALLOC;
-    if (nseg > 1 && !ckp.alloc(FB * 9 * (size_t)nseg)) return BD_ERR_ALLOC;
-    T* S = (T*)seg.p;
+    if (nseg > 1 && (!ckp.alloc(FB * 9 * (size_t)nseg) || !seg[1].alloc(FB * (size_t)(K + 5)))) return BD_ERR_ALLOC;
+    // Segment j lives in store j&1: the recompute of segment j-1 fills one
+    // store while the adjoint sweep of segment j reads the other.
+    auto Sl = [&](int m, int a0) {   // p^m in the store of the segment starting at a0
+        return (T*)seg[(a0 / K) & 1].p + (size_t)(m - (a0 - 4)) * NF;
+    };
     T* Rr = (T*)rring.p;
-    auto Sl = [&](int m, int a0) { return S + (size_t)(m - (a0 - 4)) * NF; };   // p^m in the store of segment a0
     auto R1 = [&](int m) { return Rr + (size_t)(((m % 3) + 3) % 3) * NF; };
     auto R2 = [&](int m) { return Rr + (size_t)(3 + ((m % 3) + 3) % 3) * NF; };
     auto CK = [&](int j, int k) { return (T*)ckp.p + ((size_t)j * 9 + k) * NF; };
@@ -224,29 +273,32 @@ static int adjoint_t(const bd_westervelt_params* prm, int Nxy, int Nz, int Nt, i
         for (int k = 0; k <= Nt; ++k) { times[k] = t; t = t + prm->dt; }
     }
     WvObjective ob{obj->cx, obj->cy, obj->cz, obj->radius * obj->radius};
-    auto step = [&](int m, int a0, double* ep) -> int {
+    auto step = [&](int m, int a0, double* ep, hipStream_t s_) -> int {
         WvStepPtrs q;
         q.pn = Sl(m + 1, a0); q.p = Sl(m, a0); q.pm1 = Sl(m - 1, a0); q.pm2 = Sl(m - 2, a0);
         q.pm3 = Sl(m - 3, a0); q.pm4 = Sl(m - 4, a0);
         q.r1 = R1(m); q.r1m1 = R1(m - 1); q.r1m2 = R1(m - 2);
         q.r2 = R2(m); q.r2m1 = R2(m - 1); q.r2m2 = R2(m - 2);
         return wv_forward_step(prm, sizeof(T) == 4 ? BD_F32 : BD_F64, Nxy, Nz, q, m, times[(size_t)m + 1], amp, phase,
-                               ep, &ob, st);
+                               ep, &ob, s_);
+    };
+    auto cp = [&](void* dst, const void* src, hipStream_t s_) {
+        return hipMemcpyAsync(dst, src, FB, hipMemcpyDeviceToDevice, s_);
     };
     auto save_ck = [&](int j, int a0) -> int {  // state at step a0 (store of segment a0) into checkpoint j
-        for (int k = 0; k < 5; ++k) BD_HIP_CHECK(hipMemcpyAsync(CK(j, k), Sl(a0 - k, a0), FB, hipMemcpyDeviceToDevice, st));
-        BD_HIP_CHECK(hipMemcpyAsync(CK(j, 5), R1(a0 - 1), FB, hipMemcpyDeviceToDevice, st));
-        BD_HIP_CHECK(hipMemcpyAsync(CK(j, 6), R1(a0 - 2), FB, hipMemcpyDeviceToDevice, st));
-        BD_HIP_CHECK(hipMemcpyAsync(CK(j, 7), R2(a0 - 1), FB, hipMemcpyDeviceToDevice, st));
-        BD_HIP_CHECK(hipMemcpyAsync(CK(j, 8), R2(a0 - 2), FB, hipMemcpyDeviceToDevice, st));
+        for (int k = 0; k < 5; ++k) BD_HIP_CHECK(cp(CK(j, k), Sl(a0 - k, a0), st));
+        BD_HIP_CHECK(cp(CK(j, 5), R1(a0 - 1), st));
+        BD_HIP_CHECK(cp(CK(j, 6), R1(a0 - 2), st));
+        BD_HIP_CHECK(cp(CK(j, 7), R2(a0 - 1), st));
+        BD_HIP_CHECK(cp(CK(j, 8), R2(a0 - 2), st));
         return BD_OK;
     };
-    auto load_ck = [&](int j, int a0) -> int {  // checkpoint j (state at a0) into the store and R rings
-        for (int k = 0; k < 5; ++k) BD_HIP_CHECK(hipMemcpyAsync(Sl(a0 - k, a0), CK(j, k), FB, hipMemcpyDeviceToDevice, st));
-        BD_HIP_CHECK(hipMemcpyAsync(R1(a0 - 1), CK(j, 5), FB, hipMemcpyDeviceToDevice, st));
-        BD_HIP_CHECK(hipMemcpyAsync(R1(a0 - 2), CK(j, 6), FB, hipMemcpyDeviceToDevice, st));
-        BD_HIP_CHECK(hipMemcpyAsync(R2(a0 - 1), CK(j, 7), FB, hipMemcpyDeviceToDevice, st));
-        BD_HIP_CHECK(hipMemcpyAsync(R2(a0 - 2), CK(j, 8), FB, hipMemcpyDeviceToDevice, st));
+    auto load_ck = [&](int j, int a0, hipStream_t s_) -> int {  // checkpoint j (state at a0) into its store and the R rings
+        for (int k = 0; k < 5; ++k) BD_HIP_CHECK(cp(Sl(a0 - k, a0), CK(j, k), s_));
+        BD_HIP_CHECK(cp(R1(a0 - 1), CK(j, 5), s_));
+        BD_HIP_CHECK(cp(R1(a0 - 2), CK(j, 6), s_));
+        BD_HIP_CHECK(cp(R2(a0 - 1), CK(j, 7), s_));
+        BD_HIP_CHECK(cp(R2(a0 - 2), CK(j, 8), s_));
         return BD_OK;
     };
     auto now = [&]() {
@@ -260,13 +312,11 @@ static int adjoint_t(const bd_westervelt_params* prm, int Nxy, int Nz, int Nt, i
     for (int j = 0; j < nseg; ++j) {
         const int a0 = j * K, b = std::min(Nt, a0 + K);
         if (j > 0) {
-            // shift the last five levels to the head of the store (ascending
-            // destination slots: a source slot is read before anything lands on it)
-            for (int k = 4; k >= 0; --k)
-                BD_HIP_CHECK(hipMemcpyAsync(Sl(a0 - k, a0), Sl(a0 - k, a0 - K), FB, hipMemcpyDeviceToDevice, st));
+            // the last five levels of segment j-1 head the store of segment j
+            for (int k = 4; k >= 0; --k) BD_HIP_CHECK(cp(Sl(a0 - k, a0), Sl(a0 - k, a0 - K), st));
         }
         if (nseg > 1 && j > 0) { int rc = save_ck(j, a0); if (rc) return rc; }
-        for (int m = a0; m < b; ++m) { int rc = step(m, a0, (double*)epart.p); if (rc) return rc; }
+        for (int m = a0; m < b; ++m) { int rc = step(m, a0, (double*)epart.p, st); if (rc) return rc; }
     }
     hipLaunchKernelGGL(adj_sum_kernel, dim3(1), dim3(256), 0, st, (const double*)epart.p, (size_t)nb * Nt,
                        (double*)jsum.p);
@@ -311,41 +361,87 @@ static int adjoint_t(const bd_westervelt_params* prm, int Nxy, int Nz, int Nt, i
     A.N = N; A.Nz = Nz; A.sponge_w = prm->sponge_cells; A.sponge_alpha = prm->sponge_alpha;
     A.ocx = obj->cx; A.ocy = obj->cy; A.ocz = obj->cz; A.or2 = obj->radius * obj->radius;
     A.wT = obj->w_target; A.wO = obj->w_outside;
+    // 64 or 128 columns wide, whichever leaves fewer idle lanes (BD_ADJ_TILE overrides)
+    int tile = kAdjDefaultTile;
+    {
+        const int nin = N - 6, w64 = (nin + 63) / 64 * 64 - nin, w128 = (nin + 127) / 128 * 128 - nin;
+        if (w128 <= w64) tile = kAdjWideTile;
+    }
+    if (const char* v = std::getenv("BD_ADJ_TILE")) { const int x = std::atoi(v); if (x >= 0 && x < 5) tile = x; }
+    const int kAX = kAdjTiles[tile].ax, kAY = kAdjTiles[tile].ay;
     A.ntx = (N - 6 + kAX - 1) / kAX;
     A.nty = (N - 6 + kAY - 1) / kAY;
     {
+        // target workgroup count (BD_ADJ_BLOCKS overrides it for the tile sweep)
+        int target = 1024;
+        if (const char* v = std::getenv("BD_ADJ_BLOCKS")) { const int x = std::atoi(v); if (x > 0) target = x; }
         const int nzin = Nz - 6;
-        int nzc = std::max(1, 1024 / (A.ntx * A.nty));
+        int nzc = std::max(1, target / (A.ntx * A.nty));
         int zchunk = std::max(std::min(8, nzin), (nzin + nzc - 1) / nzc);
         A.nzc = (nzin + zchunk - 1) / zchunk;
         A.zchunk = zchunk;
         A.nblocks = A.ntx * A.nty * A.nzc;
     }
-    double t_recompute = 0.0, t_backward = 0.0;
+    // recompute stream (default priority) and the events that order the two streams
+    struct Side {
+        hipStream_t s = nullptr;
+        std::vector<hipEvent_t> ev;
+        ~Side() {
+            for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } side;
+    BD_HIP_CHECK(hipStreamCreateWithFlags(&side.s, hipStreamNonBlocking));
+    side.ev.assign(4 + 2 * (size_t)nseg, nullptr);
+    for (auto& e : side.ev) BD_HIP_CHECK(hipEventCreate(&e));
+    hipEvent_t* recomputed = side.ev.data();       // [store]: its recompute finished
+    hipEvent_t* swept = side.ev.data() + 2;        // [store]: the adjoint sweep over it finished
+    hipEvent_t* sweep_t = side.ev.data() + 4;      // [2j], [2j+1]: sweep of segment j starts / ends
+    // recompute p^{a0+1..b} of segment j from its checkpoint (segment 0 from rest) on the side stream
+    auto recompute = [&](int j) -> int {
+        const int a0 = j * K, b = std::min(Nt, a0 + K), sl = j & 1;
+        hipStream_t s2 = side.s;
+        if (j + 2 < nseg) BD_HIP_CHECK(hipStreamWaitEvent(s2, swept[sl], 0));   // store last read by segment j+2
+        if (j == 0) {
+            BD_HIP_CHECK(hipMemsetAsync(Sl(-4, 0), 0, FB * 5, s2));
+            BD_HIP_CHECK(hipMemsetAsync(Rr, 0, FB * 6, s2));
+        } else {
+            int rc = load_ck(j, a0, s2);
+            if (rc) return rc;
+        }
+        for (int m = a0; m < b; ++m) { int rc = step(m, a0, nullptr, s2); if (rc) return rc; }
+        BD_HIP_CHECK(hipEventRecord(recomputed[sl], s2));
+        return BD_OK;
+    };
+    auto launch_adj = [&](hipStream_t s_) -> int {
+        switch (tile) {
+            case 0: hipLaunchKernelGGL((adj_step_kernel<T, 64, 4>), dim3(A.nblocks), dim3(256), 0, s_, A); break;
+            case 1: hipLaunchKernelGGL((adj_step_kernel<T, 64, 8>), dim3(A.nblocks), dim3(512), 0, s_, A); break;
+            case 2: hipLaunchKernelGGL((adj_step_kernel<T, 32, 8>), dim3(A.nblocks), dim3(256), 0, s_, A); break;
+            case 3: hipLaunchKernelGGL((adj_step_kernel<T, 128, 4>), dim3(A.nblocks), dim3(512), 0, s_, A); break;
+            default: hipLaunchKernelGGL((adj_step_kernel<T, 128, 8>), dim3(A.nblocks), dim3(1024), 0, s_, A); break;
+        }
+        BD_LAUNCH_CHECK();
+        return BD_OK;
+    };
+    double t_sweep = 0.0;
+    const auto p0 = std::chrono::steady_clock::now();
     {
         // lam^{Nt}
         const int last_a0 = (nseg - 1) * K;
         hipLaunchKernelGGL((adj_init_kernel<T>), dim3((unsigned)((NF + 255) / 256)), dim3(256), 0, st, Lv(Nt),
                            (const T*)Sl(Nt, last_a0), N, Nz, obj->cx, obj->cy, obj->cz, A.or2, obj->w_target,
-                           obj->w_outside);
+                           obj->w_outside, prm->sponge_cells, prm->sponge_alpha);
         BD_LAUNCH_CHECK();
     }
+    // Segments nseg-1 and nseg-2 are still in the two stores after the
+    // forward pass; every earlier segment is recomputed into the store of
+    // segment j+2 once the sweep over it is done, overlapping the sweep of j+1.
+    if (nseg > 1) BD_HIP_CHECK(hipEventRecord(recomputed[(nseg - 2) & 1], st));
     for (int j = nseg - 1; j >= 0; --j) {
-        const int a0 = j * K, b = std::min(Nt, a0 + K);
-        auto r0 = now();
-        if (j != nseg - 1) {
-            // recompute p^{a0+1..b} from the checkpoint at a0 (segment 0 starts from rest)
-            if (j == 0) {
-                BD_HIP_CHECK(hipMemsetAsync(Sl(-4, 0), 0, FB * 5, st));
-                BD_HIP_CHECK(hipMemsetAsync(Rr, 0, FB * 6, st));
-            } else {
-                int rc = load_ck(j, a0);
-                if (rc) return rc;
-            }
-            for (int m = a0; m < b; ++m) { int rc = step(m, a0, nullptr); if (rc) return rc; }
-        }
-        auto r1 = now();
-        t_recompute += ms(r0, r1);
+        const int a0 = j * K, b = std::min(Nt, a0 + K), sl = j & 1;
+        if (j != nseg - 1) BD_HIP_CHECK(hipStreamWaitEvent(st, recomputed[sl], 0));
+        BD_HIP_CHECK(hipEventRecord(sweep_t[2 * j], st));
         for (int n = b - 1; n >= std::max(a0, 1); --n) {
             A.lam1 = Lv(n + 1); A.lam0 = Lv(n);
             A.p0 = Sl(n, a0); A.p1 = Sl(n - 1, a0); A.p2 = Sl(n - 2, a0); A.p3 = Sl(n - 3, a0);
@@ -353,10 +449,21 @@ static int adjoint_t(const bd_westervelt_params* prm, int Nxy, int Nz, int Nt, i
             A.r1a = Rh1(n + 1); A.r1b = Rh1(n + 2); A.r1o = Rh1(n);
             A.r2a = Rh2(n + 1); A.r2b = Rh2(n + 2); A.r2o = Rh2(n);
             A.lsrc = (T*)lsrc.p + (size_t)n * plane;
-            hipLaunchKernelGGL((adj_step_kernel<T>), dim3(A.nblocks), dim3(kAX * kAY), 0, st, A);
-            BD_LAUNCH_CHECK();
+            int rc = launch_adj(st);
+            if (rc) return rc;
         }
-        t_backward += ms(r1, now());
+        BD_HIP_CHECK(hipEventRecord(sweep_t[2 * j + 1], st));
+        BD_HIP_CHECK(hipEventRecord(swept[sl], st));
+        // segment j-2 reuses this store: its recompute starts once this sweep is done
+        if (j - 2 >= 0) { int rc = recompute(j - 2); if (rc) return rc; }
+    }
+    BD_HIP_CHECK(hipStreamSynchronize(side.s));
+    BD_HIP_CHECK(hipStreamSynchronize(st));
+    const double t_phase = ms(p0, std::chrono::steady_clock::now());
+    for (int j = 0; j < nseg; ++j) {
+        float e = 0.f;
+        BD_HIP_CHECK(hipEventElapsedTime(&e, sweep_t[2 * j], sweep_t[2 * j + 1]));
+        t_sweep += e;
     }
     // element cell lists (host), then the gradient reduction
     const int ne = prm->arr_nx * prm->arr_ny;
@@ -383,7 +490,7 @@ static int adjoint_t(const bd_westervelt_params* prm, int Nxy, int Nz, int Nt, i
                        prm->burst_cycles, prm->burst_t0, prm->fr, gA, gP);
     BD_LAUNCH_CHECK();
     BD_HIP_CHECK(hipStreamSynchronize(st));
-    if (timings) { timings[1] = t_recompute; timings[2] = t_backward; }
+    if (timings) { timings[1] = std::max(0.0, t_phase - t_sweep); timings[2] = t_sweep; }
     return BD_OK;
 }
 

@@ -307,12 +307,19 @@ __global__ void energy_finalize_kernel(const double* epart, double* energy, int 
 // launch helpers
 // ---------------------------------------------------------------------------
 constexpr int kDefaultVariant = 0;
+constexpr int kWide128Variant = 8;
 constexpr int kDefaultBlocks = 1024;
 
 // Variant / target block count for the interior kernel; BD_WV_VARIANT and
-// BD_WV_BLOCKS override the defaults (used by the tile-shape sweep).
-static void launch_shape(int* variant, int* blocks) {
+// BD_WV_BLOCKS override the defaults (used by the tile-shape sweep).  The
+// default tile is 64 or 128 columns wide, whichever leaves fewer idle lanes
+// across the inside columns (the wider one on a tie: fewer halo columns).
+static void launch_shape(int* variant, int* blocks, int nin = 0) {
     *variant = kDefaultVariant;
+    if (nin > 0) {
+        const int w64 = (nin + 63) / 64 * 64 - nin, w128 = (nin + 127) / 128 * 128 - nin;
+        if (w128 <= w64) *variant = kWide128Variant;
+    }
     *blocks = kDefaultBlocks;
     if (const char* v = std::getenv("BD_WV_VARIANT")) {
         const int x = std::atoi(v);
@@ -397,6 +404,8 @@ static int launch_march(const WvMarchArgs<T>& a, int variant, int step_arg, hipS
         case 5: launch_variant<T, 5>(a, step_arg, st); break;
         case 6: launch_variant<T, 6>(a, step_arg, st); break;
         case 7: launch_variant<T, 7>(a, step_arg, st); break;
+        case 8: launch_variant<T, 8>(a, step_arg, st); break;
+        case 9: launch_variant<T, 9>(a, step_arg, st); break;
         default: return BD_ERR_INVALID;
     }
     BD_LAUNCH_CHECK();
@@ -463,7 +472,7 @@ static int forward_step_t(const bd_westervelt_params* prm, int Nxy, int Nz, cons
     a.r2 = (T*)q.r2; a.r2m1 = (const T*)q.r2m1; a.r2m2 = (const T*)q.r2m2;
     a.pmax = nullptr; a.epart = epart; a.focus = nullptr; a.d_step = nullptr; a.d_step_next = nullptr; a.focus_idx = -1;
     int variant, blocks;
-    launch_shape(&variant, &blocks);
+    launch_shape(&variant, &blocks, Nxy - 3 - (prm->full_domain ? 3 : 0));
     a.g = make_geom(prm, Nxy, Nz, variant, blocks);
     if (obj) { a.g.obj_on = 1; a.g.ocx = obj->cx; a.g.ocy = obj->cy; a.g.ocz = obj->cz; a.g.or2 = obj->r2; }
     a.c = make_coef<T>(prm);
@@ -481,7 +490,7 @@ int wv_forward_step(const bd_westervelt_params* prm, int dtype, int Nxy, int Nz,
 
 int wv_forward_nblocks(const bd_westervelt_params* prm, int Nxy, int Nz) {
     int variant, blocks;
-    launch_shape(&variant, &blocks);
+    launch_shape(&variant, &blocks, Nxy - 3 - (prm->full_domain ? 3 : 0));
     return make_geom(prm, Nxy, Nz, variant, blocks).nblocks;
 }
 
@@ -508,7 +517,7 @@ static int space6_t(const bd_westervelt_params* prm, void* Pnext, const void* P,
     a.r2 = (T*)R2; a.r2m1 = (const T*)R2m1; a.r2m2 = (const T*)R2m2;
     a.pmax = nullptr; a.epart = nullptr; a.focus = nullptr; a.d_step = nullptr; a.d_step_next = nullptr; a.focus_idx = -1;
     int variant, blocks;
-    launch_shape(&variant, &blocks);
+    launch_shape(&variant, &blocks, Nxy - 3 - (prm->full_domain ? 3 : 0));
     a.g = make_geom(prm, Nxy, Nz, variant, blocks);
     a.c = make_coef<T>(prm);
     set_fused_z<T>(a, prm, 0, nullptr, nullptr);
@@ -675,7 +684,7 @@ extern "C" int bd_westervelt_plan_create(const bd_westervelt_params* prm, int dt
     auto* pl = new bd_westervelt_plan();
     pl->prm = *prm; pl->dtype = dtype; pl->Nxy = Nxy; pl->Nz = Nz; pl->max_steps = max_steps;
     int blocks = 0;
-    launch_shape(&pl->variant, &blocks);
+    launch_shape(&pl->variant, &blocks, Nxy - 3 - (prm->full_domain ? 3 : 0));
     pl->g = make_geom(prm, Nxy, Nz, pl->variant, blocks);
     const int lo = prm->full_domain ? 3 : 0;
     pl->focus_idx = -1;

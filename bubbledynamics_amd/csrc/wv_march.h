@@ -550,6 +550,7 @@ struct WvVariant { int tx, ty, ry, pf; };
 constexpr WvVariant kWvVariants[] = {
     {64, 4, 1, 0}, {64, 4, 1, 1}, {64, 4, 2, 1}, {64, 8, 1, 1},
     {64, 8, 2, 1}, {64, 4, 4, 1}, {64, 8, 2, 0}, {32, 8, 2, 1},
+    {128, 4, 1, 0}, {128, 2, 2, 0},
 };
 constexpr int kWvNumVariants = sizeof(kWvVariants) / sizeof(kWvVariants[0]);
 

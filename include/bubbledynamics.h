@@ -305,8 +305,11 @@ typedef struct bd_adjoint_objective {
  * (host) and, when grad_amp/grad_phase are non-NULL (device, arr_nx*arr_ny
  * doubles each), dJ/dA_e and dJ/dphi_e from the discrete adjoint.  Forward
  * levels are checkpointed every checkpoint_every steps and recomputed per
- * segment (checkpoint_every >= n_steps keeps them all).  timings_ms (host,
- * nullable) receives forward / recompute / backward wall times.  Synchronous. */
+ * segment (checkpoint_every >= n_steps keeps them all); the recompute of a
+ * segment runs on a second stream while the adjoint sweeps the segment after
+ * it.  timings_ms (host, nullable) receives three wall times: the forward
+ * pass, the part of the backward phase outside the adjoint sweep (recompute
+ * not hidden under it), and the adjoint sweep.  Synchronous. */
 int bd_westervelt_adjoint(const bd_westervelt_params* prm, int dtype, int Nxy, int Nz, int n_steps,
                           int checkpoint_every, const double* elem_amp, const double* elem_phase,
                           const bd_adjoint_objective* obj, double* J, double* grad_amp,
