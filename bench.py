@@ -94,6 +94,8 @@ def gilmore_figures() -> dict:
     R0, amp, ph, fr = gm.config3_inputs(1_000_000, 3)
     b = gm.BubbleBatch(R0, amp, ph, fr)
     gm.BubbleBatch(R0[:4096], amp[:4096], ph[:4096], fr).run(p, BD_GM_DP45, 2e-6, 1e-13)   # warm-up
+    # launch shape from the autotuner (probe: every bubble capped at 500 attempts)
+    tune, t_tune = _timed(lambda: b.autotune(p, BD_GM_DP45, 20e-6, 1e-13, rtol=1e-8, atol=1e-12))
     o64, t64 = _timed(lambda: b.run(p, BD_GM_DP45, 20e-6, 1e-13, rtol=1e-8, atol=1e-12))
     s64 = int(o64["n_acc"].sum().item())
     rej64 = int(o64["n_rej"].sum().item())
@@ -108,7 +110,8 @@ def gilmore_figures() -> dict:
     sc = int(oc["n_acc"].sum().item())
     return {
         "dp45_1e6_fp64": {"bubble_steps_per_s": s64 / t64, "accepted_steps": s64, "rejected": rej64,
-                          "seconds": t64, "collapsed_frac": float((o64["t_collapse"] > 0).float().mean().item())},
+                          "seconds": t64, "collapsed_frac": float((o64["t_collapse"] > 0).float().mean().item()),
+                          "autotuned_shape": tune, "autotune_seconds": t_tune},
         "dp45_1e6_split": {"bubble_steps_per_s": s32 / t32, "accepted_steps": s32, "seconds": t32,
                            "rmax_rel_err_vs_fp64": {"median": float(np.median(rel)),
                                                     "p90": float(np.quantile(rel, 0.9)),

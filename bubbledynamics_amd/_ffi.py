@@ -80,7 +80,7 @@ class GilmoreBatch(C.Structure):
     """Mirror of bd_gilmore_batch."""
     _fields_ = [("method", C.c_int32), ("precision", C.c_int32), ("n", C.c_int32),
                 ("out_every", C.c_int32), ("n_out", C.c_int32), ("block", C.c_int32),
-                ("chunk", C.c_int32), ("pad0", C.c_int32),
+                ("chunk", C.c_int32), ("waves", C.c_int32),
                 ("freq", C.c_double * 3), ("t_end", C.c_double), ("dt", C.c_double),
                 ("rtol", C.c_double), ("atol", C.c_double), ("split_ratio", C.c_double),
                 ("max_steps", C.c_longlong),
@@ -149,6 +149,9 @@ SIGNATURES: dict[str, tuple] = {
     "bd_gilmore_threshold_sweep": (_I, [C.POINTER(GilmoreParams), _VP, _VP, _I, _I, _I, _VP]),
     "bd_gilmore_workspace_bytes": (C.c_size_t, []),
     "bd_gilmore_integrate": (_I, [C.POINTER(GilmoreParams), C.POINTER(GilmoreBatch), _VP, _VP]),
+    "bd_gilmore_autotune": (_I, [C.POINTER(GilmoreParams), C.POINTER(GilmoreBatch), C.c_int, C.c_longlong, _VP,
+                                 C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_double)]),
 }
 
 _lib = None

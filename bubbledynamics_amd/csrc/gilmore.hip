@@ -484,14 +484,36 @@ static GmConsts make_consts(const bd_gilmore_params* g) {
     return K;
 }
 
-static int persistent_blocks(int block) {
+// Persistent grid: `waves` wavefronts per SIMD (4 SIMDs per CU) on every CU.
+static int persistent_blocks(int block, int waves = 8) {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
         hipDeviceProp_t pr;
         if (hipGetDeviceProperties(&pr, dev) == hipSuccess && pr.multiProcessorCount > 0) cus = pr.multiProcessorCount;
     }
-    // enough waves per SIMD to hide FP64 latency; bounded by the work
-    return cus * std::max(1, 2048 / block);
+    return cus * std::max(1, waves * 4 * kWave / block);
+}
+
+// Launch shape of the batch integrator: workgroup size, bubbles per queue
+// grab, wavefronts per SIMD of the persistent grid.  bd_gilmore_autotune
+// measures the candidates and records the fastest per device, method and
+// precision; bd_gilmore_integrate uses it for fields left at 0.
+struct GmShape { int block, chunk, waves; };
+constexpr GmShape kGmDefaultShape = {256, 1, 8};
+constexpr int kGmMaxDevices = 64;
+static GmShape g_gm_tuned[kGmMaxDevices][2][2];
+static bool g_gm_has_tuned[kGmMaxDevices][2][2];
+
+static GmShape gm_shape(const bd_gilmore_batch* b) {
+    GmShape s = kGmDefaultShape;
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < kGmMaxDevices &&
+        g_gm_has_tuned[dev][b->method][b->precision])
+        s = g_gm_tuned[dev][b->method][b->precision];
+    if (b->block > 0) s.block = b->block;
+    if (b->chunk > 0) s.chunk = b->chunk;
+    if (b->waves > 0) s.waves = b->waves;
+    return s;
 }
 
 // one counter per device for the drop-in sweep (its signature has no workspace)
@@ -575,18 +597,10 @@ extern "C" int bd_gilmore_threshold_sweep(const bd_gilmore_params* prm, double* 
 
 extern "C" size_t bd_gilmore_workspace_bytes(void) { return 256; }
 
-extern "C" int bd_gilmore_integrate(const bd_gilmore_params* prm, const bd_gilmore_batch* b, void* workspace,
-                                    void* stream) {
-    int rc = check_gilmore(prm);
-    if (rc) return rc;
-    if (!b || !workspace || b->n <= 0 || !b->R0 || !b->amp || !b->phase || !b->Rmax || !b->t_collapse)
+static int gm_launch(const bd_gilmore_params* prm, const bd_gilmore_batch* b, void* workspace, GmShape shp,
+                     hipStream_t st) {
+    if (shp.block % kWave != 0 || shp.block > 256 || shp.block <= 0 || shp.chunk <= 0 || shp.waves <= 0)
         return BD_ERR_INVALID;
-    if (b->method != BD_GM_RK4 && b->method != BD_GM_DP45) return BD_ERR_INVALID;
-    if (b->precision != BD_GM_FP64 && b->precision != BD_GM_SPLIT) return BD_ERR_INVALID;
-    if (!(b->dt > 0) || !(b->t_end > 0)) return BD_ERR_INVALID;
-    if (b->R_out && b->n_out <= 0) return BD_ERR_INVALID;
-    if (b->method == BD_GM_DP45 && (!(b->rtol > 0) || !(b->atol >= 0))) return BD_ERR_INVALID;
-    hipStream_t st = as_stream(stream);
     GmBatchArgs a;
     a.K = make_consts(prm);
     if (b->method == BD_GM_DP45) { a.K.abs_eps = b->atol; a.K.rel_eps = b->rtol; }
@@ -595,7 +609,7 @@ extern "C" int bd_gilmore_integrate(const bd_gilmore_params* prm, const bd_gilmo
     a.n = b->n;
     a.out_every = b->out_every;
     a.n_out = b->R_out ? b->n_out : 0;
-    a.chunk = b->chunk > 0 ? b->chunk : 1;
+    a.chunk = shp.chunk;
     for (int k = 0; k < 3; ++k) a.w[k] = 2.0 * M_PI * b->freq[k];
     a.t_end = b->t_end;
     a.dt = b->dt;
@@ -607,9 +621,8 @@ extern "C" int bd_gilmore_integrate(const bd_gilmore_params* prm, const bd_gilmo
     a.counter = (int*)workspace;
     hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, st, a.counter);
     BD_LAUNCH_CHECK();
-    const int block = b->block > 0 ? b->block : 256;
-    if (block % kWave != 0 || block > 256) return BD_ERR_INVALID;
-    int blocks = persistent_blocks(block);
+    const int block = shp.block;
+    int blocks = persistent_blocks(block, shp.waves);
     const long long need = ((long long)b->n + (long long)a.chunk - 1) / a.chunk;
     blocks = (int)std::min<long long>(blocks, (need + block - 1) / block);
     const dim3 grid(std::max(1, blocks)), blk(block);
@@ -622,5 +635,96 @@ extern "C" int bd_gilmore_integrate(const bd_gilmore_params* prm, const bd_gilmo
     else
         hipLaunchKernelGGL((gm_batch_kernel<BD_GM_DP45, BD_GM_SPLIT>), grid, blk, 0, st, a);
     BD_LAUNCH_CHECK();
+    return BD_OK;
+}
+
+static int check_batch(const bd_gilmore_params* prm, const bd_gilmore_batch* b) {
+    int rc = check_gilmore(prm);
+    if (rc) return rc;
+    if (!b || b->n <= 0 || !b->R0 || !b->amp || !b->phase || !b->Rmax || !b->t_collapse || !b->n_acc || !b->n_rej)
+        return BD_ERR_INVALID;
+    if (b->method != BD_GM_RK4 && b->method != BD_GM_DP45) return BD_ERR_INVALID;
+    if (b->precision != BD_GM_FP64 && b->precision != BD_GM_SPLIT) return BD_ERR_INVALID;
+    if (!(b->dt > 0) || !(b->t_end > 0)) return BD_ERR_INVALID;
+    if (b->R_out && b->n_out <= 0) return BD_ERR_INVALID;
+    if (b->method == BD_GM_DP45 && (!(b->rtol > 0) || !(b->atol >= 0))) return BD_ERR_INVALID;
+    if (b->block < 0 || b->chunk < 0 || b->waves < 0) return BD_ERR_INVALID;
+    return BD_OK;
+}
+
+extern "C" int bd_gilmore_integrate(const bd_gilmore_params* prm, const bd_gilmore_batch* b, void* workspace,
+                                    void* stream) {
+    int rc = check_batch(prm, b);
+    if (rc) return rc;
+    if (!workspace) return BD_ERR_INVALID;
+    return gm_launch(prm, b, workspace, gm_shape(b), as_stream(stream));
+}
+
+// Autotuner: times every candidate launch shape on the first `sample`
+// bubbles of the batch (each capped at probe_steps attempts, so every lane
+// has work for the whole run and the rate measures steady-state throughput),
+// keeps the fastest for this device / method / precision and returns it.
+// The integrator's results do not depend on the launch shape.
+extern "C" int bd_gilmore_autotune(const bd_gilmore_params* prm, const bd_gilmore_batch* batch, int sample,
+                                   long long probe_steps, void* stream, int32_t* best_block, int32_t* best_chunk,
+                                   int32_t* best_waves, double* best_rate) {
+    int rc = check_batch(prm, batch);
+    if (rc) return rc;
+    if (sample < 0 || probe_steps < 0) return BD_ERR_INVALID;
+    int dev = 0;
+    BD_HIP_CHECK(hipGetDevice(&dev));
+    if (dev >= kGmMaxDevices) return BD_ERR_UNSUPPORTED;
+    hipStream_t st = as_stream(stream);
+    const int ns = std::min(batch->n, sample > 0 ? sample : 1 << 20);
+    bd_gilmore_batch b = *batch;
+    b.n = ns;
+    b.R_out = nullptr; b.n_out = 0; b.out_every = 0;
+    b.max_steps = probe_steps > 0 ? probe_steps : 500;
+    struct Scratch {
+        void* p = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Scratch() {
+            if (p) (void)hipFree(p);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
+    } sc;
+    BD_HIP_CHECK(hipMalloc(&sc.p, (size_t)ns * 32 + 256));
+    BD_HIP_CHECK(hipEventCreate(&sc.e0));
+    BD_HIP_CHECK(hipEventCreate(&sc.e1));
+    char* base = (char*)sc.p;
+    b.Rmax = (double*)base; b.t_collapse = (double*)(base + (size_t)ns * 8);
+    b.n_acc = (long long*)(base + (size_t)ns * 16); b.n_rej = (long long*)(base + (size_t)ns * 24);
+    void* counter = base + (size_t)ns * 32;
+    const GmShape cands[] = {
+        {64, 1, 2}, {64, 1, 4}, {64, 1, 8}, {128, 1, 2}, {128, 1, 4}, {128, 1, 8},
+        {256, 1, 2}, {256, 1, 4}, {256, 1, 8}, {64, 4, 8}, {128, 4, 8}, {256, 4, 8},
+    };
+    // warm-up with the default shape, and the step total (the same for every shape)
+    rc = gm_launch(prm, &b, counter, kGmDefaultShape, st);
+    if (rc) return rc;
+    std::vector<long long> acc((size_t)ns);
+    BD_HIP_CHECK(hipMemcpyAsync(acc.data(), b.n_acc, (size_t)ns * 8, hipMemcpyDeviceToHost, st));
+    BD_HIP_CHECK(hipStreamSynchronize(st));
+    double steps = 0.0;
+    for (long long v : acc) steps += (double)v;
+    GmShape best = kGmDefaultShape;
+    double best_ms = 1e300;
+    for (const GmShape& c : cands) {
+        BD_HIP_CHECK(hipEventRecord(sc.e0, st));
+        rc = gm_launch(prm, &b, counter, c, st);
+        if (rc) return rc;
+        BD_HIP_CHECK(hipEventRecord(sc.e1, st));
+        BD_HIP_CHECK(hipEventSynchronize(sc.e1));
+        float t = 0.f;
+        BD_HIP_CHECK(hipEventElapsedTime(&t, sc.e0, sc.e1));
+        if (t < best_ms) { best_ms = t; best = c; }
+    }
+    g_gm_tuned[dev][b.method][b.precision] = best;
+    g_gm_has_tuned[dev][b.method][b.precision] = true;
+    if (best_block) *best_block = best.block;
+    if (best_chunk) *best_chunk = best.chunk;
+    if (best_waves) *best_waves = best.waves;
+    if (best_rate) *best_rate = steps / (best_ms * 1e-3);
     return BD_OK;
 }

@@ -87,9 +87,8 @@ class BubbleBatch:
         self.workspace = torch.zeros(int(_ffi.load().bd_gilmore_workspace_bytes()), dtype=torch.uint8,
                                      device="cuda")
 
-    def run(self, params: GilmoreParams, method: int, t_end: float, dt: float, *, precision=BD_GM_FP64,
-            rtol=1e-8, atol=1e-12, split_ratio=0.2, out_every=0, n_out=0, max_steps=0, block=0,
-            chunk=0) -> dict:
+    def _batch(self, method, t_end, dt, precision, rtol, atol, split_ratio, out_every, n_out, max_steps,
+               block, chunk, waves):
         n = self.n
         dev = "cuda"
         out = {
@@ -102,7 +101,7 @@ class BubbleBatch:
         b = GilmoreBatch()
         b.method, b.precision, b.n = method, precision, n
         b.out_every, b.n_out = out_every, n_out
-        b.block, b.chunk = block, chunk
+        b.block, b.chunk, b.waves = block, chunk, waves
         for k in range(3):
             b.freq[k] = self.freq[k]
         b.t_end, b.dt, b.rtol, b.atol, b.split_ratio = t_end, dt, rtol, atol, split_ratio
@@ -111,8 +110,27 @@ class BubbleBatch:
         b.R_out = out["R_out"].data_ptr() if n_out else None
         b.Rmax, b.t_collapse = out["Rmax"].data_ptr(), out["t_collapse"].data_ptr()
         b.n_acc, b.n_rej = out["n_acc"].data_ptr(), out["n_rej"].data_ptr()
+        return b, out
+
+    def run(self, params: GilmoreParams, method: int, t_end: float, dt: float, *, precision=BD_GM_FP64,
+            rtol=1e-8, atol=1e-12, split_ratio=0.2, out_every=0, n_out=0, max_steps=0, block=0,
+            chunk=0, waves=0) -> dict:
+        """Integrates every bubble; block / chunk / waves left at 0 take the
+        autotuned shape for this method and precision (or the default)."""
+        b, out = self._batch(method, t_end, dt, precision, rtol, atol, split_ratio, out_every, n_out,
+                             max_steps, block, chunk, waves)
         call("bd_gilmore_integrate", C.byref(params), C.byref(b), _vp(self.workspace), _stream())
         return out
+
+    def autotune(self, params: GilmoreParams, method: int, t_end: float, dt: float, *, precision=BD_GM_FP64,
+                 rtol=1e-8, atol=1e-12, split_ratio=0.2, sample=0, probe_steps=0) -> dict:
+        """Times the candidate launch shapes on this batch's first `sample`
+        bubbles and records the fastest for later runs; returns it."""
+        b, _ = self._batch(method, t_end, dt, precision, rtol, atol, split_ratio, 0, 0, 0, 0, 0, 0)
+        blk, ch, wv, rate = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
+        call("bd_gilmore_autotune", C.byref(params), C.byref(b), int(sample), int(probe_steps), _stream(),
+             C.byref(blk), C.byref(ch), C.byref(wv), C.byref(rate))
+        return {"block": blk.value, "chunk": ch.value, "waves": wv.value, "probe_rate": rate.value}
 
 
 def seeded_uniforms(seed: int, n: int) -> np.ndarray:

@@ -268,7 +268,7 @@ typedef struct bd_gilmore_batch {
     int32_t n_out;            /* samples per bubble in R_out (NaN after collapse)      */
     int32_t block;            /* workgroup size (0 = tuned default)                    */
     int32_t chunk;            /* bubbles taken per queue grab (0 = tuned default)      */
-    int32_t pad0;
+    int32_t waves;            /* wavefronts per SIMD of the persistent grid (0 = tuned default) */
     double freq[3];
     double t_end, dt;         /* dt: RK4 step, DP45 first step                         */
     double rtol, atol;        /* DP45 tolerances                                        */
@@ -289,6 +289,16 @@ size_t bd_gilmore_workspace_bytes(void);
  * next bubble from a device counter as soon as its previous one finished. */
 int bd_gilmore_integrate(const bd_gilmore_params* prm, const bd_gilmore_batch* batch,
                          void* workspace, void* stream);
+/* Autotuner for the launch shape of bd_gilmore_integrate (no counterpart in
+ * the reference): times block in {64,128,256} x waves per SIMD in {2,4,8}
+ * (and bubbles per grab in {1,4} at 8 waves) on the first `sample` bubbles of
+ * the batch (0 = up to 2^20), each capped at probe_steps attempts (0 = 500),
+ * and records the fastest for this device, method and precision; later
+ * integrate calls use it for block / chunk / waves left at 0.  Returns the
+ * choice and its rate (bubble-steps/s).  Allocates scratch; synchronous. */
+int bd_gilmore_autotune(const bd_gilmore_params* prm, const bd_gilmore_batch* batch, int sample,
+                        long long probe_steps, void* stream, int32_t* block, int32_t* chunk,
+                        int32_t* waves, double* rate);
 
 /* ---- adjoint: phased-array gradient (no reference counterpart) ---------- */
 

@@ -105,3 +105,24 @@ def test_queue_scheduling_does_not_change_results():
         torch.cuda.synchronize()
         assert torch.equal(o["Rmax"], base["Rmax"])
         assert torch.equal(o["n_acc"], base["n_acc"])
+
+
+def test_autotune_picks_a_candidate_and_shape_does_not_change_results():
+    # the autotuner measures launch shapes on a capped probe and records the
+    # fastest; runs with the tuned shape and with explicit waves match bitwise
+    p = gm.workload_params()
+    R0, amp, ph, fr = gm.config3_inputs(4096, 3)
+    b = gm.BubbleBatch(R0, amp, ph, fr)
+    base = b.run(p, BD_GM_DP45, 4e-6, 1e-13, block=256, chunk=1, waves=8)
+    t = b.autotune(p, BD_GM_DP45, 4e-6, 1e-13, sample=4096, probe_steps=300)
+    assert t["block"] in (64, 128, 256) and t["chunk"] in (1, 4) and t["waves"] in (2, 4, 8)
+    assert t["probe_rate"] > 0
+    tuned = b.run(p, BD_GM_DP45, 4e-6, 1e-13)
+    for waves in (2, 4):
+        o = b.run(p, BD_GM_DP45, 4e-6, 1e-13, block=128, waves=waves)
+        torch.cuda.synchronize()
+        assert torch.equal(o["Rmax"], base["Rmax"]) and torch.equal(o["n_acc"], base["n_acc"])
+    torch.cuda.synchronize()
+    assert torch.equal(tuned["Rmax"], base["Rmax"])
+    assert torch.equal(tuned["t_collapse"], base["t_collapse"])
+    assert torch.equal(tuned["n_rej"], base["n_rej"])
