@@ -89,18 +89,21 @@ def gilmore_figures() -> dict:
     FP32/FP64 split; 1,024 bubbles, RK4 dt = 0.1 ns, 20 us, FP64 with R(t) every 1 ns."""
     import numpy as np
     from bubbledynamics_amd import gilmore as gm
-    from bubbledynamics_amd._ffi import BD_GM_DP45, BD_GM_RK4, BD_GM_SPLIT
+    from bubbledynamics_amd._ffi import BD_GM_DP45, BD_GM_RK4, BD_GM_SCHED_SLICED, BD_GM_SPLIT
     p = gm.workload_params()
     R0, amp, ph, fr = gm.config3_inputs(1_000_000, 3)
     b = gm.BubbleBatch(R0, amp, ph, fr)
     gm.BubbleBatch(R0[:4096], amp[:4096], ph[:4096], fr).run(p, BD_GM_DP45, 2e-6, 1e-13)   # warm-up
     # launch shape from the autotuner (probe: every bubble capped at 500 attempts)
     tune, t_tune = _timed(lambda: b.autotune(p, BD_GM_DP45, 20e-6, 1e-13, rtol=1e-8, atol=1e-12))
-    o64, t64 = _timed(lambda: b.run(p, BD_GM_DP45, 20e-6, 1e-13, rtol=1e-8, atol=1e-12))
+    # sliced schedule: passes of 128 and 2048 attempts, the rest longest-first
+    o64, t64 = _timed(lambda: b.run(p, BD_GM_DP45, 20e-6, 1e-13, rtol=1e-8, atol=1e-12,
+                                    schedule=BD_GM_SCHED_SLICED))
+    _, t64_input = _timed(lambda: b.run(p, BD_GM_DP45, 20e-6, 1e-13, rtol=1e-8, atol=1e-12))
     s64 = int(o64["n_acc"].sum().item())
     rej64 = int(o64["n_rej"].sum().item())
     o32, t32 = _timed(lambda: b.run(p, BD_GM_DP45, 20e-6, 1e-13, precision=BD_GM_SPLIT, rtol=1e-8,
-                                    atol=1e-12, split_ratio=0.2))
+                                    atol=1e-12, split_ratio=0.2, schedule=BD_GM_SCHED_SLICED))
     s32 = int(o32["n_acc"].sum().item())
     rel = (o32["Rmax"] / o64["Rmax"] - 1).abs().cpu().numpy()
     flips = float(((o32["t_collapse"] > 0) != (o64["t_collapse"] > 0)).float().mean().item())
@@ -111,7 +114,8 @@ def gilmore_figures() -> dict:
     return {
         "dp45_1e6_fp64": {"bubble_steps_per_s": s64 / t64, "accepted_steps": s64, "rejected": rej64,
                           "seconds": t64, "collapsed_frac": float((o64["t_collapse"] > 0).float().mean().item()),
-                          "autotuned_shape": tune, "autotune_seconds": t_tune},
+                          "autotuned_shape": tune, "autotune_seconds": t_tune, "schedule": "sliced (128, 2048, rest)",
+                          "input_order_bubble_steps_per_s": s64 / t64_input},
         "dp45_1e6_split": {"bubble_steps_per_s": s32 / t32, "accepted_steps": s32, "seconds": t32,
                            "rmax_rel_err_vs_fp64": {"median": float(np.median(rel)),
                                                     "p90": float(np.quantile(rel, 0.9)),

@@ -86,7 +86,8 @@ class GilmoreBatch(C.Structure):
                 ("max_steps", C.c_longlong),
                 ("R0", C.c_void_p), ("amp", C.c_void_p), ("phase", C.c_void_p),
                 ("R_out", C.c_void_p), ("Rmax", C.c_void_p), ("t_collapse", C.c_void_p),
-                ("n_acc", C.c_void_p), ("n_rej", C.c_void_p), ("reserved", C.c_double * 4)]
+                ("n_acc", C.c_void_p), ("n_rej", C.c_void_p), ("schedule", C.c_int32), ("slice", C.c_int32),
+                ("reserved", C.c_double * 3)]
 
 
 class BioheatParams(C.Structure):
@@ -113,6 +114,8 @@ BD_GM_RK4 = 0
 BD_GM_DP45 = 1
 BD_GM_FP64 = 0
 BD_GM_SPLIT = 1
+BD_GM_SCHED_INPUT = 0
+BD_GM_SCHED_SLICED = 1
 
 _VP = C.c_void_p
 _I = C.c_int

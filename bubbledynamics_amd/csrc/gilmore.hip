@@ -239,6 +239,14 @@ __device__ __forceinline__ double step_factor(double err) { return fmax(0.1, 0.9
 // ---------------------------------------------------------------------------
 // batch integrator (persistent, per-lane refill)
 // ---------------------------------------------------------------------------
+// Integrator state of a bubble suspended at the end of a time slice.
+struct GmLane {
+    double y[4], k1[4];
+    double t, h, rmax, tc, fac_max;
+    long long acc, rej, s_rk;
+    int j_out, flags;           // flags: 1 = k1 valid, 2 = at the start of a step
+};
+
 struct GmBatchArgs {
     GmConsts K;
     int method, precision, n, out_every, n_out, chunk;
@@ -249,6 +257,15 @@ struct GmBatchArgs {
     double* R_out; double* Rmax; double* t_collapse;
     long long* n_acc; long long* n_rej;
     int* counter;
+    const int* order;        // queue position -> bubble index, nullable (input order)
+    // time slicing: a bubble still running after `slice` attempts in this
+    // launch is suspended into state[i], with its predicted remaining
+    // attempts in cost[i] (-1 once it is finished); resume=1 continues
+    // bubbles from state[i]
+    GmLane* state;
+    long long* cost;
+    long long slice;
+    int resume;
 };
 
 template <int METHOD, int PREC>
@@ -262,6 +279,7 @@ __global__ void __launch_bounds__(256) gm_batch_kernel(GmBatchArgs a) {
     double y[4] = {0, 0, 0, 0}, k1[4] = {0, 0, 0, 0};
     bool k1_ok = false, at_start = true;
     long long acc = 0, rej = 0, s_rk = 0, nsteps_rk = 0;
+    long long sl = 0;               // attempts of the current bubble in this launch
     int j_out = 0;
     while (true) {
         if (!have) {
@@ -270,19 +288,30 @@ __global__ void __launch_bounds__(256) gm_batch_kernel(GmBatchArgs a) {
                 end = min(next + a.chunk, a.n);
                 if (next >= a.n) break;
             }
-            i = next++;
+            const int qpos = next++;
+            i = a.order ? a.order[qpos] : qpos;
             have = true;
+            sl = 0;
             R0 = a.R0[i];
             Pin0 = K.m64.P0 + K.m64.S2 / R0;
 #pragma unroll
             for (int k = 0; k < 3; ++k) { d.w[k] = a.w[k]; d.A[k] = a.amp[3 * i + k]; d.ph[k] = a.phase[3 * i + k]; }
-            y[0] = R0; y[1] = y[2] = y[3] = 0.0;
-            t = 0.0; h = a.dt; rmax = R0; tc = -1.0; fac_max = 5.0;
-            k1_ok = false; at_start = true;
-            acc = rej = 0; s_rk = 0;
             nsteps_rk = llrint(a.t_end / a.dt);
-            j_out = 0;
-            if (a.R_out && a.n_out > 0) a.R_out[(size_t)i * a.n_out + j_out++] = R0;
+            if (a.resume) {
+                const GmLane& L = a.state[i];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) { y[c] = L.y[c]; k1[c] = L.k1[c]; }
+                t = L.t; h = L.h; rmax = L.rmax; tc = L.tc; fac_max = L.fac_max;
+                acc = L.acc; rej = L.rej; s_rk = L.s_rk; j_out = L.j_out;
+                k1_ok = (L.flags & 1) != 0; at_start = (L.flags & 2) != 0;
+            } else {
+                y[0] = R0; y[1] = y[2] = y[3] = 0.0;
+                t = 0.0; h = a.dt; rmax = R0; tc = -1.0; fac_max = 5.0;
+                k1_ok = false; at_start = true;
+                acc = rej = 0; s_rk = 0;
+                j_out = 0;
+                if (a.R_out && a.n_out > 0) a.R_out[(size_t)i * a.n_out + j_out++] = R0;
+            }
         }
         bool done = false;
         const bool use32 = PREC == BD_GM_SPLIT && y[0] >= a.split_ratio * R0;
@@ -333,6 +362,7 @@ __global__ void __launch_bounds__(256) gm_batch_kernel(GmBatchArgs a) {
                 if (a.max_steps > 0 && acc + rej >= a.max_steps) done = true;
             }
         }
+        ++sl;
         if (done) {
             if (a.R_out)
                 for (; j_out < a.n_out; ++j_out) a.R_out[(size_t)i * a.n_out + j_out] = NAN;
@@ -340,6 +370,18 @@ __global__ void __launch_bounds__(256) gm_batch_kernel(GmBatchArgs a) {
             a.t_collapse[i] = tc;
             if (a.n_acc) a.n_acc[i] = acc;
             if (a.n_rej) a.n_rej[i] = rej;
+            if (a.cost) a.cost[i] = -1;
+            have = false;
+        } else if (a.slice > 0 && sl >= a.slice) {
+            GmLane& L = a.state[i];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) { L.y[c] = y[c]; L.k1[c] = k1[c]; }
+            L.t = t; L.h = h; L.rmax = rmax; L.tc = tc; L.fac_max = fac_max;
+            L.acc = acc; L.rej = rej; L.s_rk = s_rk; L.j_out = j_out;
+            L.flags = (k1_ok ? 1 : 0) | (at_start ? 2 : 0);
+            // remaining attempts, extrapolated from the rate so far
+            const double frac = t > 0.0 ? (a.t_end - t) / t : 1e6;
+            a.cost[i] = (long long)fmin(4.0e18, fmax(1.0, (double)(acc + rej) * frac));
             have = false;
         }
     }
@@ -597,8 +639,67 @@ extern "C" int bd_gilmore_threshold_sweep(const bd_gilmore_params* prm, double* 
 
 extern "C" size_t bd_gilmore_workspace_bytes(void) { return 256; }
 
+// Queue order of the second pass: quarter-octave buckets of the predicted
+// remaining attempts, most expensive bucket first, finished bubbles left
+// out (order inside a bucket is the scatter's arrival order; the results do
+// not depend on it).  hist[kGmBuckets] receives the number of bubbles queued.
+constexpr int kGmBuckets = 128;
+__device__ __forceinline__ int gm_bucket(long long cost) {
+    const int k = (int)(4.0 * log2((double)cost + 1.0));
+    return k < 0 ? 0 : (k >= kGmBuckets ? kGmBuckets - 1 : k);
+}
+__global__ void gm_zero_buckets(int* hist) {
+    if (threadIdx.x <= kGmBuckets) hist[threadIdx.x] = 0;
+}
+__global__ void gm_bucket_hist(const long long* cost, int n, int* hist) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && cost[i] >= 0) atomicAdd(&hist[gm_bucket(cost[i])], 1);
+}
+__global__ void gm_bucket_starts(int* hist) {   // counts -> start positions, top bucket first
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int run = 0;
+    for (int k = kGmBuckets - 1; k >= 0; --k) { const int c = hist[k]; hist[k] = run; run += c; }
+    hist[kGmBuckets] = run;
+}
+__global__ void gm_bucket_scatter(const long long* cost, int n, int* cursor, int* order) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && cost[i] >= 0) order[atomicAdd(&cursor[gm_bucket(cost[i])], 1)] = i;
+}
+
+// Scratch of the sliced schedule, one growing buffer per device, held until
+// the process ends: state[n], cost[n], order[n], bucket table.
+struct GmSched { GmLane* state; long long* cost; int* order; int* hist; };
+static bool gm_sched_scratch(int n, GmSched* out) {
+    static void* buf[kGmMaxDevices] = {nullptr};
+    static size_t cap[kGmMaxDevices] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kGmMaxDevices) return false;
+    const size_t s_state = sizeof(GmLane) * (size_t)n, s_cost = sizeof(long long) * (size_t)n;
+    const size_t s_order = sizeof(int) * (size_t)n, s_hist = sizeof(int) * (kGmBuckets + 1);
+    const size_t need = s_state + s_cost + s_order + s_hist;
+    if (cap[dev] < need) {
+        if (buf[dev]) { (void)hipDeviceSynchronize(); (void)hipFree(buf[dev]); buf[dev] = nullptr; cap[dev] = 0; }
+        if (hipMalloc(&buf[dev], need) != hipSuccess) return false;
+        cap[dev] = need;
+    }
+    char* p = (char*)buf[dev];
+    out->state = (GmLane*)p;
+    out->cost = (long long*)(p + s_state);
+    out->order = (int*)(p + s_state + s_cost);
+    out->hist = (int*)(p + s_state + s_cost + s_order);
+    return true;
+}
+
+struct GmPass {                  // order / slicing of one launch of the batch kernel
+    const int* order = nullptr;
+    GmLane* state = nullptr;
+    long long* cost = nullptr;
+    long long slice = 0;
+    int resume = 0;
+};
+
 static int gm_launch(const bd_gilmore_params* prm, const bd_gilmore_batch* b, void* workspace, GmShape shp,
-                     hipStream_t st) {
+                     hipStream_t st, const GmPass& pass = GmPass()) {
     if (shp.block % kWave != 0 || shp.block > 256 || shp.block <= 0 || shp.chunk <= 0 || shp.waves <= 0)
         return BD_ERR_INVALID;
     GmBatchArgs a;
@@ -619,6 +720,11 @@ static int gm_launch(const bd_gilmore_params* prm, const bd_gilmore_batch* b, vo
     a.R_out = b->R_out; a.Rmax = b->Rmax; a.t_collapse = b->t_collapse;
     a.n_acc = b->n_acc; a.n_rej = b->n_rej;
     a.counter = (int*)workspace;
+    a.order = pass.order;
+    a.state = pass.state;
+    a.cost = pass.cost;
+    a.slice = pass.slice;
+    a.resume = pass.resume;
     hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, st, a.counter);
     BD_LAUNCH_CHECK();
     const int block = shp.block;
@@ -657,7 +763,43 @@ extern "C" int bd_gilmore_integrate(const bd_gilmore_params* prm, const bd_gilmo
     int rc = check_batch(prm, b);
     if (rc) return rc;
     if (!workspace) return BD_ERR_INVALID;
-    return gm_launch(prm, b, workspace, gm_shape(b), as_stream(stream));
+    if (b->schedule != BD_GM_SCHED_INPUT && b->schedule != BD_GM_SCHED_SLICED) return BD_ERR_INVALID;
+    if (b->slice < 0) return BD_ERR_INVALID;
+    hipStream_t st = as_stream(stream);
+    const GmShape shp = gm_shape(b);
+    if (b->schedule == BD_GM_SCHED_INPUT) return gm_launch(prm, b, workspace, shp, st);
+    // Passes of growing slices (S, 16 S, then to the end): every pass runs the
+    // bubbles still unfinished, largest predicted remaining work first, so a
+    // long bubble the previous ranking underrated is moved up after one slice.
+    GmSched sc;
+    if (!gm_sched_scratch(b->n, &sc)) return BD_ERR_ALLOC;
+    const long long S = b->slice > 0 ? b->slice : 128;
+    const long long slices[3] = {S, 16 * S, 0};
+    const unsigned g = (unsigned)((b->n + 255) / 256);
+    int m = b->n;
+    for (int pass = 0; pass < 3 && m > 0; ++pass) {
+        bd_gilmore_batch bp = *b;
+        bp.n = m;
+        GmPass pp;
+        pp.order = pass > 0 ? sc.order : nullptr;
+        pp.state = sc.state;
+        pp.cost = sc.cost;
+        pp.slice = slices[pass];
+        pp.resume = pass > 0;
+        rc = gm_launch(prm, &bp, workspace, shp, st, pp);
+        if (rc) return rc;
+        if (slices[pass] == 0) break;
+        // rank the unfinished bubbles (cost >= 0) by predicted remaining attempts
+        hipLaunchKernelGGL(gm_zero_buckets, dim3(1), dim3(kGmBuckets + 1), 0, st, sc.hist);
+        hipLaunchKernelGGL(gm_bucket_hist, dim3(g), dim3(256), 0, st, (const long long*)sc.cost, b->n, sc.hist);
+        hipLaunchKernelGGL(gm_bucket_starts, dim3(1), dim3(1), 0, st, sc.hist);
+        hipLaunchKernelGGL(gm_bucket_scatter, dim3(g), dim3(256), 0, st, (const long long*)sc.cost, b->n, sc.hist,
+                           sc.order);
+        BD_LAUNCH_CHECK();
+        BD_HIP_CHECK(hipMemcpyAsync(&m, sc.hist + kGmBuckets, sizeof(int), hipMemcpyDeviceToHost, st));
+        BD_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    return BD_OK;
 }
 
 // Autotuner: times every candidate launch shape on the first `sample`

@@ -13,8 +13,8 @@ import numpy as np
 import torch
 
 from . import _ffi
-from ._ffi import (BD_GM_DP45, BD_GM_FP64, BD_GM_RK4, BD_GM_SPLIT, BD_GM_TAB_DP1980,
-                   BD_GM_TAB_REFERENCE, GilmoreBatch, GilmoreParams, call)
+from ._ffi import (BD_GM_DP45, BD_GM_FP64, BD_GM_RK4, BD_GM_SCHED_INPUT, BD_GM_SCHED_SLICED,  # noqa: F401
+                   BD_GM_SPLIT, BD_GM_TAB_DP1980, BD_GM_TAB_REFERENCE, GilmoreBatch, GilmoreParams, call)
 
 
 def reference_params() -> GilmoreParams:
@@ -88,7 +88,7 @@ class BubbleBatch:
                                      device="cuda")
 
     def _batch(self, method, t_end, dt, precision, rtol, atol, split_ratio, out_every, n_out, max_steps,
-               block, chunk, waves):
+               block, chunk, waves, schedule=0, slice_=0):
         n = self.n
         dev = "cuda"
         out = {
@@ -102,6 +102,8 @@ class BubbleBatch:
         b.method, b.precision, b.n = method, precision, n
         b.out_every, b.n_out = out_every, n_out
         b.block, b.chunk, b.waves = block, chunk, waves
+        b.schedule = schedule
+        b.slice = slice_
         for k in range(3):
             b.freq[k] = self.freq[k]
         b.t_end, b.dt, b.rtol, b.atol, b.split_ratio = t_end, dt, rtol, atol, split_ratio
@@ -114,11 +116,12 @@ class BubbleBatch:
 
     def run(self, params: GilmoreParams, method: int, t_end: float, dt: float, *, precision=BD_GM_FP64,
             rtol=1e-8, atol=1e-12, split_ratio=0.2, out_every=0, n_out=0, max_steps=0, block=0,
-            chunk=0, waves=0) -> dict:
+            chunk=0, waves=0, schedule=BD_GM_SCHED_INPUT, slice_attempts=0) -> dict:
         """Integrates every bubble; block / chunk / waves left at 0 take the
-        autotuned shape for this method and precision (or the default)."""
+        autotuned shape for this method and precision (or the default);
+        schedule=BD_GM_SCHED_SLICED starts the predicted-longest bubbles first."""
         b, out = self._batch(method, t_end, dt, precision, rtol, atol, split_ratio, out_every, n_out,
-                             max_steps, block, chunk, waves)
+                             max_steps, block, chunk, waves, schedule, slice_attempts)
         call("bd_gilmore_integrate", C.byref(params), C.byref(b), _vp(self.workspace), _stream())
         return out
 

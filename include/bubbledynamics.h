@@ -280,8 +280,22 @@ typedef struct bd_gilmore_batch {
     double* t_collapse;       /* n, -1 if no collapse before t_end                       */
     long long* n_acc;         /* n, accepted steps                                      */
     long long* n_rej;         /* n, rejected attempts (DP45)                            */
-    double reserved[4];
+    int32_t schedule;         /* BD_GM_SCHED_*: order in which the queue hands out bubbles */
+    int32_t slice;            /* BD_GM_SCHED_SLICED: attempts of the first pass (0 = 128) */
+    double reserved[3];
 } bd_gilmore_batch;
+
+/* Queue order of bd_gilmore_integrate (the results do not depend on it):
+ * INPUT hands bubbles out in input order.  SLICED runs passes of `slice`,
+ * 16*slice and then unlimited attempts per bubble: a bubble still running at
+ * the end of a slice is suspended with its state saved, and the next pass
+ * continues the suspended bubbles ordered by their extrapolated remaining
+ * attempts (quarter-octave buckets, most expensive first), so the longest
+ * bubbles do not start late and hold up the end of the run.  SLICED keeps a
+ * per-device scratch (about 160 bytes per bubble) until the process ends and
+ * synchronises the stream between passes. */
+#define BD_GM_SCHED_INPUT  0
+#define BD_GM_SCHED_SLICED 1
 
 /* Bytes of device workspace bd_gilmore_integrate needs (queue counter). */
 size_t bd_gilmore_workspace_bytes(void);
