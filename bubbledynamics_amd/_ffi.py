@@ -17,6 +17,7 @@ BD_ERR_ALLOC = 3
 BD_ERR_HIP = 4
 BD_ERR_NONFINITE = 5
 BD_ERR_UNSUPPORTED = 6
+BD_ERR_IO = 7
 BD_F32 = 0
 BD_F64 = 1
 BD_BC_MUR = 0
@@ -146,6 +147,11 @@ SIGNATURES: dict[str, tuple] = {
                                    C.POINTER(_D), _VP, _VP, C.POINTER(_D), _VP]),
     "bd_temperature_square": (_I, [C.POINTER(BioheatParams), _I, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _I, _VP]),
     "bd_bioheat_step": (_I, [C.POINTER(BioheatParams), _I, _VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _VP]),
+    "bd_q3d_write": (_I, [C.c_char_p, _I, _VP, _I, _I, _I, _VP]),
+    "bd_q3d_read_expand": (_I, [C.c_char_p, _I, _VP, _I, _I, _VP]),
+    "bd_mirror_expand": (_I, [_I, _VP, _VP, _I, _I, _VP]),
+    "bd_bioheat_run": (_I, [C.POINTER(BioheatParams), _I, _VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _I,
+                            C.POINTER(C.c_int), _VP]),
     "bd_spiral_reference": (_I, [_D, _D, _D, _D, _D, _D, _D, _I, C.POINTER(_I), C.POINTER(_I), _I]),
     "bd_spiral_dwell": (_I, [_I, _D, _D, C.POINTER(_I), C.POINTER(_I)]),
     "bd_gilmore_reference_params": (None, [C.POINTER(GilmoreParams)]),

@@ -101,14 +101,15 @@ class BioheatRun:
         self.steps = 0
 
     def step(self, sx: int, sy: int, n: int = 1) -> None:
+        """n substeps with the source shifted by (sx, sy) cells (launched by the library)."""
         st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-        lib = _ffi.load()
-        for _ in range(n):
-            rc = lib.bd_bioheat_step(C.byref(self.p), self.code, _vp(self.T[1 - self.cur]), _vp(self.T[self.cur]),
-                                     _vp(self.Q), _vp(self.TD), _vp(self.TDc), self.nxy, self.nz, sx, sy, st)
-            _ffi.check("bd_bioheat_step", rc)
-            self.cur = 1 - self.cur
-            self.steps += 1
+        which = C.c_int(self.cur)
+        rc = _ffi.load().bd_bioheat_run(C.byref(self.p), self.code, _vp(self.T[0]), _vp(self.T[1]), _vp(self.Q),
+                                        _vp(self.TD), _vp(self.TDc), self.nxy, self.nz, sx, sy, n, C.byref(which),
+                                        st)
+        _ffi.check("bd_bioheat_run", rc)
+        self.cur = which.value
+        self.steps += n
 
     def temperature(self) -> torch.Tensor:
         return self.T[self.cur].double() + self.p.T_off

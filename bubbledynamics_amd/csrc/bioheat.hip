@@ -4,13 +4,15 @@
 //
 // Kernel (bh_march): a workgroup owns a 64 x 4 tile of columns and a chunk of
 // z planes; each thread marches z with a 3-deep register queue of its own
-// column, takes the x/y neighbours straight from the cache hierarchy (radius
-// 1: they share cache lines with neighbouring lanes and rows), and streams Q
-// (shifted), TD and the optional Kahan compensation once.  Fields may hold
-// T - T_off so that FP32 resolves small temperature rises.
+// column, the plane with a one-cell x/y halo is staged in LDS (double
+// buffered, one barrier per plane), and Q (shifted), TD and the optional
+// Kahan compensation are streamed once; every load of plane z+1 is issued
+// before plane z is computed.  Fields may hold T - T_off so
+// that FP32 resolves small temperature rises.
 #include "common.h"
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 #include <vector>
 
@@ -32,68 +34,99 @@ struct BhArgs {
 
 constexpr int kBX = 64, kBY = 4;
 
+// Loads of one plane for one thread, issued a plane ahead of their use.
+template <typename T>
+struct BhPlane { T zp, hx, hy, q, td, tdc; };
+
 template <typename T>
 __global__ void __launch_bounds__(kBX * kBY) bh_march(BhArgs<T> a) {
+    __shared__ T s[2][kBY + 2][kBX + 2];
     const int lb = xcd_remap((int)blockIdx.x, a.nblocks);
     const int zc = lb % a.nzc, t2 = lb / a.nzc;
     const int bx = t2 % a.ntx, by = t2 / a.ntx;
-    const int ix = 1 + bx * kBX + (int)threadIdx.x % kBX;
-    const int iy = 1 + by * kBY + (int)threadIdx.x / kBX;
+    const int tx = (int)threadIdx.x % kBX, ty = (int)threadIdx.x / kBX;
+    const int ix = 1 + bx * kBX + tx;
+    const int iy = 1 + by * kBY + ty;
     const int N = a.Nxy;
-    if (ix > N - 2 || iy > N - 2) return;          // no barrier below: early exit is safe
+    const bool inr = ix < N && iy < N;              // a grid cell (loads)
+    const bool act = ix <= N - 2 && iy <= N - 2;    // an interior cell (update)
     const size_t plane = (size_t)N * N;
     const uint32_t ES = sizeof(T);
+    const uint32_t pbytes = (uint32_t)(plane * ES);
     const int z0 = 1 + zc * a.zchunk;
     const int z1 = min(z0 + a.zchunk, a.Nz - 1);
-    const uint32_t c = (uint32_t)iy * (uint32_t)N + (uint32_t)ix;
+    const uint32_t c = inr ? (uint32_t)iy * (uint32_t)N + (uint32_t)ix : 0u;
+    const uint32_t o = c * ES;
     const int ixa = ix - a.shift_x, iya = iy - a.shift_y;
-    const bool q_in = ixa >= 0 && ixa < N && iya >= 0 && iya < N;          // T:76
+    const bool q_in = act && ixa >= 0 && ixa < N && iya >= 0 && iya < N;   // T:76
     const uint32_t cq = q_in ? ((uint32_t)iya * (uint32_t)N + (uint32_t)ixa) * ES : 0xFFFFFFFFu;
-
-    T zm = a.src[(size_t)(z0 - 1) * plane + c];
-    T zc0 = a.src[(size_t)z0 * plane + c];
-    for (int z = z0; z < z1; ++z) {
+    // x / y halo cells of the tile (edge lanes / rows); a cell past the grid
+    // is read only by threads outside the interior
+    const int hxd = tx == 0 ? -1 : (tx == kBX - 1 ? 1 : 0);
+    const int hyd = ty == 0 ? -1 : (ty == kBY - 1 ? 1 : 0);
+    const bool hx_on = hxd != 0 && iy < N && ix + hxd < N;
+    const bool hy_on = hyd != 0 && ix < N && iy + hyd < N;
+    const uint32_t ohx = hx_on ? (uint32_t)((int)c + hxd) * ES : 0xFFFFFFFFu;
+    const uint32_t ohy = hy_on ? (uint32_t)((int)c + hyd * N) * ES : 0xFFFFFFFFu;
+    auto load = [&](int z, BhPlane<T>& v) {
         const size_t zo = (size_t)z * plane;
-        const rsrc_t rs = make_rsrc(a.src + zo - plane, 3 * plane * ES);
-        const uint32_t o = c * ES;
-        const T zp = bload<T>(rs, o, 2 * plane * ES);
-        const T left = bload<T>(rs, o - ES, plane * ES), right = bload<T>(rs, o + ES, plane * ES);
-        const T down = bload<T>(rs, o - N * ES, plane * ES), up = bload<T>(rs, o + N * ES, plane * ES);
+        const rsrc_t rs = make_rsrc(a.src + zo, 2 * (size_t)pbytes);
+        v.zp = inr ? bload<T>(rs, o, pbytes) : T(0);
+        v.hx = bload<T>(rs, ohx);                  // out of range offset: 0
+        v.hy = bload<T>(rs, ohy);
         // out-of-range offset (shift outside the grid) reads 0, as T:75-77
-        const T q_ac = bload<T>(make_rsrc(a.Q + zo, plane * ES), cq);
-        const rsrc_t rtd = make_rsrc(a.TD + zo, plane * ES);
-        const T td = bload<T>(rtd, o);
-        const int reg = z >= a.iz_tissue ? 1 : 0;
-        const T Tc = zc0;
-        T Tn;
-        if (a.reference_order) {
-            Tn = Tc * (T(1.0) - T(4.0) * a.kxy[reg] - T(2.0) * a.kz[reg]) + a.kxy[reg] * (left + right + up + down) +
-                 a.kz[reg] * (zp + zm) + a.dq[reg] * q_ac;
-        } else {
-            Tn = Tc * a.keep[reg] + a.kxy[reg] * ((left + right) + (up + down)) + a.kz[reg] * (zp + zm) +
-                 a.dq[reg] * q_ac;
-        }
-        if (reg && a.perf != T(0)) Tn = Tn - a.perf * (Tc - a.theta_art);
-        bstore<T>(Tn, make_rsrc(a.dst + zo, plane * ES), o);
-        T add;
-        if constexpr (std::is_same<T, float>::value) {
-            const float x = Tn + a.dose_off;               // T_c - 43
-            add = exp2f(x < 0.0f ? 2.0f * x : x) * (float)a.dt_T;
-        } else {
-            const double T_c = (Tn + a.T_off) - 273.15;   // T:88, T:96
-            add = (T_c < 43.0) ? pow(4.0, T_c - 43.0) * a.dt_T : pow(2.0, T_c - 43.0) * a.dt_T;  // T:99-102
-        }
-        if (!a.TDc) {
-            bstore<T>(td + add, rtd, o);
-        } else {
-            const rsrc_t rc = make_rsrc(a.TDc + zo, plane * ES);
-            const T comp = bload<T>(rc, o);
-            const T y = add - comp, tt = td + y;
-            bstore<T>((tt - td) - y, rc, o);
-            bstore<T>(tt, rtd, o);
+        v.q = bload<T>(make_rsrc(a.Q + zo, pbytes), cq);
+        v.td = act ? bload<T>(make_rsrc(a.TD + zo, pbytes), o) : T(0);
+        v.tdc = (act && a.TDc) ? bload<T>(make_rsrc(a.TDc + zo, pbytes), o) : T(0);
+    };
+
+    T zm = inr ? a.src[(size_t)(z0 - 1) * plane + c] : T(0);
+    T zc0 = inr ? a.src[(size_t)z0 * plane + c] : T(0);
+    BhPlane<T> cur, nxt;
+    load(z0, cur);
+    for (int z = z0; z < z1; ++z) {
+        if (z + 1 < z1) load(z + 1, nxt);
+        const int b = (z - z0) & 1;
+        s[b][ty + 1][tx + 1] = zc0;
+        if (hxd) s[b][ty + 1][tx + 1 + hxd] = cur.hx;
+        if (hyd) s[b][ty + 1 + hyd][tx + 1] = cur.hy;
+        __syncthreads();
+        if (act) {
+            const size_t zo = (size_t)z * plane;
+            const int reg = z >= a.iz_tissue ? 1 : 0;
+            const T Tc = zc0;
+            const T l = s[b][ty + 1][tx], r = s[b][ty + 1][tx + 2];
+            const T d = s[b][ty][tx + 1], u = s[b][ty + 2][tx + 1];
+            T Tn;
+            if (a.reference_order) {
+                Tn = Tc * (T(1.0) - T(4.0) * a.kxy[reg] - T(2.0) * a.kz[reg]) + a.kxy[reg] * (l + r + u + d) +
+                     a.kz[reg] * (cur.zp + zm) + a.dq[reg] * cur.q;
+            } else {
+                Tn = Tc * a.keep[reg] + a.kxy[reg] * ((l + r) + (u + d)) + a.kz[reg] * (cur.zp + zm) +
+                     a.dq[reg] * cur.q;
+            }
+            if (reg && a.perf != T(0)) Tn = Tn - a.perf * (Tc - a.theta_art);
+            bstore<T>(Tn, make_rsrc(a.dst + zo, pbytes), o);
+            T add;
+            if constexpr (std::is_same<T, float>::value) {
+                const float x = Tn + a.dose_off;               // T_c - 43
+                add = exp2f(x < 0.0f ? 2.0f * x : x) * (float)a.dt_T;
+            } else {
+                const double T_c = (Tn + a.T_off) - 273.15;   // T:88, T:96
+                add = (T_c < 43.0) ? pow(4.0, T_c - 43.0) * a.dt_T : pow(2.0, T_c - 43.0) * a.dt_T;  // T:99-102
+            }
+            const rsrc_t rtd = make_rsrc(a.TD + zo, pbytes);
+            if (!a.TDc) {
+                bstore<T>(cur.td + add, rtd, o);
+            } else {
+                const T y = add - cur.tdc, tt = cur.td + y;
+                bstore<T>((tt - cur.td) - y, make_rsrc(a.TDc + zo, pbytes), o);
+                bstore<T>(tt, rtd, o);
+            }
         }
         zm = zc0;
-        zc0 = zp;
+        zc0 = cur.zp;
+        cur = nxt;
     }
 }
 
@@ -115,7 +148,10 @@ static int launch_bh(const bd_bioheat_params* p, const T* src, T* dst, const T* 
     a.ntx = (nin + kBX - 1) / kBX;
     a.nty = (nin + kBY - 1) / kBY;
     const int nzin = Nz - 2;
-    int nzc = std::max(1, 1024 / (a.ntx * a.nty));
+    // target workgroup count (BD_BH_BLOCKS overrides it for the sweep)
+    int target = 4096;
+    if (const char* v = std::getenv("BD_BH_BLOCKS")) { const int x = std::atoi(v); if (x > 0) target = x; }
+    int nzc = std::max(1, target / (a.ntx * a.nty));
     int zchunk = std::max(std::min(8, nzin), (nzin + nzc - 1) / nzc);
     a.nzc = (nzin + zchunk - 1) / zchunk;
     a.zchunk = zchunk;
@@ -163,6 +199,31 @@ extern "C" int bd_temperature_square(const bd_bioheat_params* prm, int dtype, vo
                                 shift_x, shift_y, true, st);
     return launch_bh<double>(prm, (const double*)src, (double*)dst, (const double*)Q, (double*)TD, nullptr, Nxy,
                              Nz, shift_x, shift_y, true, st);
+}
+
+// n substeps ping-ponging between two fields, launched from here (no
+// per-substep host round trip through the caller's language).
+extern "C" int bd_bioheat_run(const bd_bioheat_params* prm, int dtype, void* T_a, void* T_b, const void* Q,
+                              void* TD, void* TD_comp, int Nxy, int Nz, int shift_x, int shift_y, int n_steps,
+                              int* which, void* stream) {
+    int rc = check_bh(prm, dtype, Nxy, Nz);
+    if (rc) return rc;
+    if (!T_a || !T_b || !Q || !TD || T_a == T_b || !which || (*which != 0 && *which != 1) || n_steps < 0)
+        return BD_ERR_INVALID;
+    hipStream_t st = as_stream(stream);
+    void* f[2] = {T_a, T_b};
+    for (int k = 0; k < n_steps; ++k) {
+        const int w = *which;
+        if (dtype == BD_F32)
+            rc = launch_bh<float>(prm, (const float*)f[w], (float*)f[1 - w], (const float*)Q, (float*)TD,
+                                  (float*)TD_comp, Nxy, Nz, shift_x, shift_y, false, st);
+        else
+            rc = launch_bh<double>(prm, (const double*)f[w], (double*)f[1 - w], (const double*)Q, (double*)TD,
+                                   (double*)TD_comp, Nxy, Nz, shift_x, shift_y, false, st);
+        if (rc) return rc;
+        *which = 1 - w;
+    }
+    return BD_OK;
 }
 
 extern "C" int bd_bioheat_step(const bd_bioheat_params* prm, int dtype, void* T_out, const void* T_in,

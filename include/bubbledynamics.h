@@ -36,6 +36,7 @@ extern "C" {
 #define BD_ERR_HIP          4  /* a HIP runtime call failed                    */
 #define BD_ERR_NONFINITE    5  /* a reduction produced NaN/Inf                  */
 #define BD_ERR_UNSUPPORTED  6  /* option combination not implemented           */
+#define BD_ERR_IO           7  /* a file could not be opened, read or written   */
 
 #define BD_F32 0
 #define BD_F64 1
@@ -203,6 +204,13 @@ int bd_bioheat_step(const bd_bioheat_params* prm, int dtype, void* T_out, const 
                     const void* Q, void* TD, void* TD_comp, int Nxy, int Nz,
                     int shift_x, int shift_y, void* stream);
 
+/* n_steps substeps of bd_bioheat_step at one source shift, alternating
+ * between the fields T_a and T_b: *which (0 or 1) names the field holding the
+ * current temperature on entry and on return.  Asynchronous on the stream. */
+int bd_bioheat_run(const bd_bioheat_params* prm, int dtype, void* T_a, void* T_b, const void* Q,
+                   void* TD, void* TD_comp, int Nxy, int Nz, int shift_x, int shift_y, int n_steps,
+                   int* which, void* stream);
+
 /* The reference's continuous square-spiral motion (T:141-173, T:323-409):
  * per time step the integer source shift (shift_x, shift_y).  start is the
  * initial focus coordinate (x = y = start; the reference uses
@@ -218,6 +226,25 @@ int bd_spiral_reference(double speed, double scale, double speed_init, double t_
  * with the reference's leg order: +x, +y, -x, -y, ...).  Writes n_side^2
  * integer shifts (rounded to the nearest cell). */
 int bd_spiral_dwell(int n_side, double pitch, double dxy, int* shift_x, int* shift_y);
+
+/* ---- Westervelt -> bioheat hand-off ------------------------------------- */
+
+/* Writes the heat source in the reference's Q3D file format (W:778-790):
+ * planes iz0..Nz-1 of the (quarter-domain) field Q (device, Nxy x Nxy x Nz),
+ * as float64, x fastest, then y, then z; the reference writes iz0 = 2 (the
+ * plane z = 0).  FP32 fields are widened.  Synchronous. */
+int bd_q3d_write(const char* path, int dtype, const void* Q, int Nxy, int Nz, int iz0, void* stream);
+
+/* Reads a Q3D file as the reference's thermal program does (T:271-302):
+ * (Nxy_full/2+1)^2 * Nz float64 values of a quarter domain, expanded to the
+ * full Nxy_full x Nxy_full x Nz grid (device, dtype) by the mirror
+ * (ix, iy) -> (|ix - Nxy_full/2|, |iy - Nxy_full/2|).  BD_ERR_IO if the file is
+ * missing or shorter.  Allocates a staging buffer; synchronous. */
+int bd_q3d_read_expand(const char* path, int dtype, void* Q_full, int Nxy_full, int Nz, void* stream);
+
+/* The same mirror expansion between device buffers: Q_quarter holds
+ * (Nxy_full/2+1)^2 * Nz values of the dtype. */
+int bd_mirror_expand(int dtype, const void* Q_quarter, void* Q_full, int Nxy_full, int Nz, void* stream);
 
 /* ======================================================================== */
 /*  Gilmore bubble dynamics in viscoelastic tissue                          */
