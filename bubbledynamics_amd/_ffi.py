@@ -139,6 +139,8 @@ SIGNATURES: dict[str, tuple] = {
     "bd_westervelt_plan_run": (_I, [_VP, _I, _I, _VP]),
     "bd_westervelt_plan_finalize": (_I, [_VP, _VP]),
     "bd_westervelt_plan_steps": (_I, [_VP]),
+    "bd_westervelt_plan_seek": (_I, [_VP, _I]),
+    "bd_westervelt_first_tissue_plane": (_I, [C.POINTER(WesterveltParams), _I]),
     "bd_westervelt_plan_slot": (_I, [_VP, _I, _I]),
     "bd_westervelt_plan_destroy": (_I, [_VP]),
     "bd_westervelt_plan_rerun_step": (_I, [_VP, _VP]),

@@ -861,6 +861,17 @@ extern "C" int bd_westervelt_plan_finalize(bd_westervelt_plan* pl, void* stream)
 
 extern "C" int bd_westervelt_plan_steps(const bd_westervelt_plan* pl) { return pl ? pl->steps : -1; }
 
+extern "C" int bd_westervelt_first_tissue_plane(const bd_westervelt_params* prm, int Nz) {
+    if (!prm || Nz <= 0) return -1;
+    return first_tissue_plane(prm, Nz);
+}
+
+extern "C" int bd_westervelt_plan_seek(bd_westervelt_plan* pl, int step) {
+    if (!pl || !pl->state || step < 0 || step > pl->max_steps) return BD_ERR_INVALID;
+    pl->steps = step;
+    return BD_OK;
+}
+
 extern "C" int bd_westervelt_plan_slot(const bd_westervelt_plan* pl, int which, int level) {
     if (!pl || level < 0) return -1;
     const int n = pl->steps;

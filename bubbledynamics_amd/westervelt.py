@@ -78,6 +78,10 @@ class WesterveltRun:
     def steps(self) -> int:
         return int(_ffi.load().bd_westervelt_plan_steps(self._plan))
 
+    def seek(self, step: int) -> None:
+        """The state block holds the levels of `step` (e.g. a restored copy)."""
+        call("bd_westervelt_plan_seek", self._plan, int(step))
+
     def pressure(self, level: int = 0) -> torch.Tensor:
         """p^{n-level} after n steps (a view into the state block)."""
         return self.state[_ffi.load().bd_westervelt_plan_slot(self._plan, 0, level)]

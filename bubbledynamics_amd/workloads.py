@@ -88,3 +88,60 @@ def pennes_spiral(Q: torch.Tensor, n: int, dx: float = 2e-4, dwell_s: float = 1.
         run.step(int(sx[k]), int(sy[k]), per_point)
     torch.cuda.synchronize()
     return run, time.perf_counter() - t0, per_point * len(sx)
+
+
+def reference_scenario(nxy: int = 48, nz: int = 64, settle_periods: int = 6, periods: int = 4,
+                       n_harm: int = 10, steps_per_period: int = 35, dtype=torch.float64,
+                       q3d_path: str | None = None):
+    """The reference's own heat-source computation (W:556-790) on a reduced
+    grid: two-layer water/muscle medium, quarter domain with mirrors, Mur
+    faces, cap source with a hole.
+
+    1. run `settle_periods` periods (the reference: 165, W:417) and keep the
+       state (W:600-613);
+    2. one more period accumulating the water heat source (W:617-660);
+    3. from the kept state, `periods` periods accumulating the harmonic bins
+       of the tissue slab, then Q = sum_n 2 (7.2 n f/MHz) I_n there
+       (W:663-755; running DFT bins in place of the slab re-runs and FFT);
+    4. optionally write the Q3D file (W:778-790).
+    dt is set so that a period is a whole number of steps (the reference's
+    0.01/f gives 100).  Returns (Q (nz, nxy, nxy) on the GPU, params, steps)."""
+    p = presets.reference_scaled(nxy, nz)
+    p.dt = 1.0 / (p.fr * steps_per_period)
+    n_settle = settle_periods * steps_per_period
+    n_time = periods * steps_per_period
+    total = n_settle + max(steps_per_period, n_time)
+    run = WesterveltRun(p, nxy, nz, dtype=dtype, max_steps=total, pmax=False, energy=False)
+    code = BD_F32 if dtype == torch.float32 else BD_F64
+    lib = _ffi.load()
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    run.run(n_settle)
+    snapshot = run.state.clone()
+    Q = torch.zeros((nz, nxy, nxy), dtype=dtype, device="cuda")
+    for _ in range(steps_per_period):
+        run.run(1)
+        # Q += ((p^{n+1} - p^{n-1}) / 2dt)^2 dt in water (W:642)
+        _ffi.check("acoustic_power_water", lib.bd_acoustic_power_water(
+            C.byref(p), code, C.c_void_p(Q.data_ptr()), C.c_void_p(run.pressure(0).data_ptr()),
+            C.c_void_p(run.pressure(2).data_ptr()), nxy, nz, 0, st))
+    _ffi.check("acoustic_power_water", lib.bd_acoustic_power_water(
+        C.byref(p), code, C.c_void_p(Q.data_ptr()), C.c_void_p(run.pressure(0).data_ptr()),
+        C.c_void_p(run.pressure(2).data_ptr()), nxy, nz, 1, st))
+    # restart from the kept state (W:674-703)
+    run.state.copy_(snapshot)
+    run.seek(n_settle)
+    iz_t = int(lib.bd_westervelt_first_tissue_plane(C.byref(p), nz))
+    H = torch.zeros((2 * n_harm, nz - iz_t, nxy, nxy), dtype=dtype, device="cuda")
+    for i in range(n_time):
+        run.run(1)
+        _ffi.check("harmonics", lib.bd_harmonics_accumulate(code, C.c_void_p(H.data_ptr()),
+                                                            C.c_void_p(run.pressure(0).data_ptr()), nxy, nz, iz_t,
+                                                            i, n_time, periods, n_harm, 0, st))
+    _ffi.check("harmonics_q", lib.bd_harmonics_q(C.byref(p), code, C.c_void_p(Q.data_ptr()),
+                                                C.c_void_p(H.data_ptr()), nxy, nz, iz_t, n_time, n_harm, 7.2, 0,
+                                                st))
+    if q3d_path is not None:
+        _ffi.check("q3d_write", lib.bd_q3d_write(str(q3d_path).encode(), code, C.c_void_p(Q.data_ptr()), nxy, nz,
+                                                2, st))
+    torch.cuda.synchronize()
+    return Q, p, {"settle": n_settle, "water": steps_per_period, "harmonics": n_time, "iz_tissue": iz_t}

@@ -163,6 +163,16 @@ int bd_westervelt_plan_finalize(bd_westervelt_plan* plan, void* stream);
 /* Steps run so far, and the ring slot holding p^{n-level} (level 0 = newest)
  * or R1/R2 levels (which = 0: P, 1: R1, 2: R2). */
 int bd_westervelt_plan_steps(const bd_westervelt_plan* plan);
+/* First plane iz with z_min + iz*dz >= water_tissue (the medium switch of
+ * W:212; the reference's iz_tissue = 1202, W:674), Nz if none, -1 on bad
+ * arguments. */
+int bd_westervelt_first_tissue_plane(const bd_westervelt_params* prm, int Nz);
+
+/* Declares that the bound state holds the levels of step `step` (0..max_steps)
+ * in their ring slots, as a plan run to that step leaves them -- e.g. a copy
+ * of such a state restored into the bound block; the next run continues at
+ * t = t^step (the restart of W:663-704). */
+int bd_westervelt_plan_seek(bd_westervelt_plan* plan, int step);
 int bd_westervelt_plan_slot(const bd_westervelt_plan* plan, int which, int level);
 int bd_westervelt_plan_destroy(bd_westervelt_plan* plan);
 /* Launches the step kernel of the next step again (same arguments as the
