@@ -16,6 +16,7 @@
 // lanes.  State, error norm and controller are FP64; with BD_GM_SPLIT the
 // stage right-hand sides run in FP32 while R/R0 >= split_ratio.
 #include "common.h"
+#include <cstdio>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -869,4 +870,24 @@ extern "C" int bd_gilmore_autotune(const bd_gilmore_params* prm, const bd_gilmor
     if (best_waves) *best_waves = best.waves;
     if (best_rate) *best_rate = steps / (best_ms * 1e-3);
     return BD_OK;
+}
+
+// Threshold table in the reference's output format (G:580-606): a header
+// "fr1;fr2;fr3 --> R0;<R0 values>", one line per frequency triplet with its
+// N_R0 thresholds, and a closing "time (s);<seconds>" line, numbers in the
+// default stream format (6 significant digits, as printf %g).
+extern "C" int bd_gilmore_sweep_write(const char* path, const double* fr_combo, int n_combo,
+                                      const double* results, int n_r0, double min_r0, double step_r0,
+                                      double seconds) {
+    if (!path || !fr_combo || !results || n_combo <= 0 || n_r0 <= 0) return BD_ERR_INVALID;
+    FILE* f = std::fopen(path, "w");
+    if (!f) return BD_ERR_IO;
+    std::fprintf(f, "fr1;fr2;fr3 --> R0");
+    for (int i = 0; i < n_r0; ++i) std::fprintf(f, ";%g", min_r0 + i * step_r0);
+    for (int c = 0; c < n_combo; ++c) {
+        std::fprintf(f, "\n%g;%g;%g", fr_combo[3 * c], fr_combo[3 * c + 1], fr_combo[3 * c + 2]);
+        for (int i = 0; i < n_r0; ++i) std::fprintf(f, ";%g", results[(size_t)c * n_r0 + i]);
+    }
+    std::fprintf(f, "\ntime (s);%g", seconds);
+    return std::fclose(f) == 0 ? BD_OK : BD_ERR_IO;
 }

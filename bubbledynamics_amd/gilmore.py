@@ -74,6 +74,35 @@ def threshold_sweep(params: GilmoreParams, fr_combo: np.ndarray, n_r0: int) -> n
     return res.cpu().numpy()
 
 
+def threshold_sweep_full(params: GilmoreParams, fr_combo: np.ndarray, n_r0: int, chunks: int = 10,
+                         log=None) -> tuple[np.ndarray, float]:
+    """The reference's whole sweep (G:498-573): every triplet x n_r0 radii.
+    Run as `chunks` launches over strided triplet subsets (each mixes low- and
+    high-frequency triplets, so every launch is throughput bound), reporting
+    progress between launches.  Returns (results[combo * n_r0 + i_r0], seconds)."""
+    import time
+    fr = np.ascontiguousarray(fr_combo, dtype=np.float64).reshape(-1, 3)
+    out = np.empty(len(fr) * n_r0)
+    t0 = time.perf_counter()
+    for k in range(chunks):
+        idx = np.arange(k, len(fr), chunks)
+        res = threshold_sweep(params, fr[idx], n_r0).reshape(len(idx), n_r0)
+        out.reshape(len(fr), n_r0)[idx] = res
+        if log:
+            log(f"sweep chunk {k + 1}/{chunks}: {len(idx)} triplets, {time.perf_counter() - t0:.1f} s")
+    return out, time.perf_counter() - t0
+
+
+def write_sweep_table(path: str, fr_combo: np.ndarray, results: np.ndarray, n_r0: int, min_r0: float,
+                      step_r0: float, seconds: float) -> None:
+    """Threshold table in the reference's format (G:580-606)."""
+    fr = np.ascontiguousarray(fr_combo, dtype=np.float64).reshape(-1)
+    res = np.ascontiguousarray(results, dtype=np.float64).reshape(-1)
+    dp = C.POINTER(C.c_double)
+    call("bd_gilmore_sweep_write", str(path).encode(), fr.ctypes.data_as(dp), len(fr) // 3,
+         res.ctypes.data_as(dp), n_r0, min_r0, step_r0, seconds)
+
+
 class BubbleBatch:
     """A batch of bubbles with per-bubble R0 and three-tone drive amplitudes and
     phases on shared frequencies; outputs live on the GPU."""

@@ -335,6 +335,13 @@ typedef struct bd_gilmore_batch {
 #define BD_GM_SCHED_SLICED 1
 
 /* Bytes of device workspace bd_gilmore_integrate needs (queue counter). */
+/* Writes a threshold table (host arrays: results[combo * n_r0 + i_r0] as
+ * bd_gilmore_threshold_sweep returns it) in the reference's output format
+ * (G:580-606): header "fr1;fr2;fr3 --> R0;R0...", one ';'-separated line
+ * per triplet, closing line "time (s);seconds"; numbers as printf %g. */
+int bd_gilmore_sweep_write(const char* path, const double* fr_combo, int n_combo, const double* results,
+                           int n_r0, double min_r0, double step_r0, double seconds);
+
 size_t bd_gilmore_workspace_bytes(void);
 /* Integrates a batch with a persistent work-queue kernel: each lane takes the
  * next bubble from a device counter as soon as its previous one finished. */

@@ -29,3 +29,19 @@ def test_workload_distributions():
 
 def test_frequency_table_host_matches_oracle(orc):
     np.testing.assert_array_equal(gm.freq_table(), orc.freq_table())
+
+
+def test_sweep_table_format(tmp_path):
+    # the reference's table layout (G:580-606), numbers in the default stream
+    # format; known strings for the reference R0 grid and tone table
+    from bubbledynamics_amd import gilmore as gm
+    fr = gm.freq_table()[[0, 27774]]
+    res = np.array([[676000.0, 1e7, -1.0], [1878000.0, 1562000.0, 2.5e-3]])
+    path = tmp_path / "sweep.dat"
+    gm.write_sweep_table(path, fr, res, 3, 1e-7, 1e-7, 12.5)
+    lines = path.read_text().split("\n")
+    assert lines[0] == "fr1;fr2;fr3 --> R0;1e-07;2e-07;3e-07"
+    assert lines[1] == "20000;20000;20000;676000;1e+07;-1"
+    assert lines[2] == "5e+06;5e+06;5e+06;1.878e+06;1.562e+06;0.0025"
+    assert lines[3] == "time (s);12.5"
+    assert len(lines) == 4
