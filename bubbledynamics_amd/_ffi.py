@@ -18,6 +18,8 @@ BD_ERR_HIP = 4
 BD_ERR_NONFINITE = 5
 BD_ERR_UNSUPPORTED = 6
 BD_ERR_IO = 7
+BD_SLICE_XZ = 0
+BD_SLICE_XY = 1
 BD_F32 = 0
 BD_F64 = 1
 BD_BC_MUR = 0
@@ -149,6 +151,7 @@ SIGNATURES: dict[str, tuple] = {
                                    C.POINTER(_D), _VP, _VP, C.POINTER(_D), _VP]),
     "bd_temperature_square": (_I, [C.POINTER(BioheatParams), _I, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _I, _VP]),
     "bd_bioheat_step": (_I, [C.POINTER(BioheatParams), _I, _VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _VP]),
+    "bd_write_slice": (_I, [C.c_char_p, _I, _VP, _I, _I, _I, _I, _I, _I, _I, _I, C.c_double, _VP]),
     "bd_q3d_write": (_I, [C.c_char_p, _I, _VP, _I, _I, _I, _VP]),
     "bd_q3d_read_expand": (_I, [C.c_char_p, _I, _VP, _I, _I, _VP]),
     "bd_mirror_expand": (_I, [_I, _VP, _VP, _I, _I, _VP]),

@@ -114,6 +114,18 @@ class BioheatRun:
     def temperature(self) -> torch.Tensor:
         return self.T[self.cur].double() + self.p.T_off
 
+    def write_slice(self, path, plane: int, fixed: int, r0: int, r1: int, c0: int, c1: int,
+                    field: str = "T") -> None:
+        """A slice of T (kelvin) or of the dose in the reference's text format
+        (T:412-447); plane BD_SLICE_XZ or BD_SLICE_XY."""
+        if field == "T":
+            t, off = self.T[self.cur], self.p.T_off
+        else:
+            t, off = (self.dose().to(self.dtype) if self.TDc is not None else self.TD), 0.0
+        t = t.contiguous()
+        _ffi.call("bd_write_slice", str(path).encode(), self.code, _vp(t), self.nxy, self.nz, plane, fixed,
+                  r0, r1, c0, c1, off, C.c_void_p(torch.cuda.current_stream().cuda_stream))
+
     def dose(self) -> torch.Tensor:
         d = self.TD.double()
         return d - self.TDc.double() if self.TDc is not None else d

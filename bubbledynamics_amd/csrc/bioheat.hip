@@ -10,6 +10,7 @@
 // before plane z is computed.  Fields may hold T - T_off so
 // that FP32 resolves small temperature rises.
 #include "common.h"
+#include <cstdio>
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -310,4 +311,42 @@ extern "C" int bd_spiral_dwell(int n_side, double pitch, double dxy, int* sx, in
         ++len;
     }
     return BD_OK;
+}
+
+// Field slices in the reference's text format (T:412-447): a header line,
+// then one line per row, each value followed by ';', numbers as the default
+// stream format (printf %g).  XZ: rows iz in [r0, r1), columns ix in [c0, c1)
+// at iy = fixed ("Z=rows;X=columns"); XY: rows iy, columns ix at iz = fixed
+// ("Y=rows;X=columns").  `offset` is added to every value (fields stored as
+// T - T_off).
+extern "C" int bd_write_slice(const char* path, int dtype, const void* field, int Nxy, int Nz, int plane, int fixed,
+                              int r0, int r1, int c0, int c1, double offset, void* stream) {
+    if (!path || !field || (dtype != BD_F32 && dtype != BD_F64) || Nxy <= 0 || Nz <= 0) return BD_ERR_INVALID;
+    if (plane != BD_SLICE_XZ && plane != BD_SLICE_XY) return BD_ERR_INVALID;
+    const int rmax = plane == BD_SLICE_XZ ? Nz : Nxy, fmax = plane == BD_SLICE_XZ ? Nxy : Nz;
+    if (r0 < 0 || r1 > rmax || r0 >= r1 || c0 < 0 || c1 > Nxy || c0 >= c1 || fixed < 0 || fixed >= fmax)
+        return BD_ERR_INVALID;
+    hipStream_t st = as_stream(stream);
+    const size_t es = dtype == BD_F32 ? 4 : 8, nc = (size_t)(c1 - c0);
+    std::vector<unsigned char> buf(nc * (size_t)(r1 - r0) * es);
+    for (int r = r0; r < r1; ++r) {
+        const size_t iz = plane == BD_SLICE_XZ ? (size_t)r : (size_t)fixed;
+        const size_t iy = plane == BD_SLICE_XZ ? (size_t)fixed : (size_t)r;
+        const size_t off = (iz * Nxy + iy) * Nxy + (size_t)c0;
+        BD_HIP_CHECK(hipMemcpyAsync(buf.data() + (size_t)(r - r0) * nc * es, (const unsigned char*)field + off * es,
+                                    nc * es, hipMemcpyDeviceToHost, st));
+    }
+    BD_HIP_CHECK(hipStreamSynchronize(st));
+    FILE* f = std::fopen(path, "w");
+    if (!f) return BD_ERR_IO;
+    std::fprintf(f, plane == BD_SLICE_XZ ? "Z=rows;X=columns" : "Y=rows;X=columns");
+    for (int r = 0; r < r1 - r0; ++r) {
+        std::fprintf(f, "\n");
+        for (size_t c = 0; c < nc; ++c) {
+            const size_t k = (size_t)r * nc + c;
+            const double v = dtype == BD_F32 ? (double)((const float*)buf.data())[k] : ((const double*)buf.data())[k];
+            std::fprintf(f, "%g;", v + offset);
+        }
+    }
+    return std::fclose(f) == 0 ? BD_OK : BD_ERR_IO;
 }

@@ -237,6 +237,18 @@ int bd_spiral_reference(double speed, double scale, double speed_init, double t_
  * integer shifts (rounded to the nearest cell). */
 int bd_spiral_dwell(int n_side, double pitch, double dxy, int* shift_x, int* shift_y);
 
+/* Writes a slice of a device field in the reference's text format
+ * (T:412-447): header, then one line per row with every value followed by
+ * ';' (printf %g).  BD_SLICE_XZ: rows iz in [r0,r1), columns ix in [c0,c1) at
+ * iy = fixed, header "Z=rows;X=columns" (the reference: iz 1200..1700,
+ * ix 325..525, iy 425); BD_SLICE_XY: rows iy, columns ix at iz = fixed,
+ * header "Y=rows;X=columns" (iz 1500).  offset is added to every value (a
+ * field stored as T - T_off).  Synchronous. */
+#define BD_SLICE_XZ 0
+#define BD_SLICE_XY 1
+int bd_write_slice(const char* path, int dtype, const void* field, int Nxy, int Nz, int plane, int fixed,
+                   int r0, int r1, int c0, int c1, double offset, void* stream);
+
 /* ---- Westervelt -> bioheat hand-off ------------------------------------- */
 
 /* Writes the heat source in the reference's Q3D file format (W:778-790):

@@ -103,3 +103,26 @@ def test_unstable_step_rejected():
     t = torch.zeros(8 ** 3, dtype=torch.float32, device="cuda")
     rc = _ffi.load().bd_bioheat_step(C.byref(p), BD_F32, vp(t), vp(t.clone()), vp(t), vp(t), None, 8, 8, 0, 0, None)
     assert rc == _ffi.BD_ERR_UNSTABLE
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_slice_writer_reference_format(tmp_path, dtype):
+    # T:412-447 layout: header, rows of '%g;' values; XZ at fixed iy, XY at fixed iz
+    from bubbledynamics_amd._ffi import BD_SLICE_XY, BD_SLICE_XZ
+    n, nz = 12, 9
+    f = (torch.arange(nz * n * n, dtype=torch.float64) / 7.0 + 300.0).reshape(nz, n, n)
+    d = f.to(dtype).to("cuda")
+    host = d.double().cpu().numpy()
+    code = BD_F64 if dtype == torch.float64 else BD_F32
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for plane, fixed, hdr in ((BD_SLICE_XZ, 5, "Z=rows;X=columns"), (BD_SLICE_XY, 4, "Y=rows;X=columns")):
+        path = tmp_path / f"s{plane}.dat"
+        _ffi.call("bd_write_slice", str(path).encode(), code, vp(d), n, nz, plane, fixed, 2, 7, 3, 9, 0.5, st)
+        lines = path.read_text().split("\n")
+        assert lines[0] == hdr and len(lines) == 1 + 5
+        for k, r in enumerate(range(2, 7)):
+            row = host[r, fixed, 3:9] if plane == BD_SLICE_XZ else host[fixed, r, 3:9]
+            assert lines[1 + k] == "".join(f"{v + 0.5:g};" for v in row)
+    rc = _ffi.load().bd_write_slice(str(tmp_path / "x.dat").encode(), code, vp(d), n, nz, BD_SLICE_XY, nz, 0, 1, 0, 1,
+                                    0.0, st)
+    assert rc == _ffi.BD_ERR_INVALID
