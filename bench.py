@@ -312,7 +312,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4),
                      "traffic": traffic,
-                     "kernel": "wv_march<float,64,4,1,0>", "kernel_us": round(k_ms * 1e3, 2),
+                     "kernel": "wv_march<float,64,4,1,0,%s>" % ("nt" if 12 * n ** 3 * 4 > 256 << 20 else "cached"), "kernel_us": round(k_ms * 1e3, 2),
                      "alg_bytes_per_launch": alg_bytes, "cells_per_launch": n ** 3,
                      "inside_cells_per_launch": cells},
         "cpu_baseline": cpu,

@@ -86,6 +86,21 @@ __device__ __forceinline__ double bload<double>(rsrc_t r, uint32_t voff, uint32_
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
 }
 
+// Load with the non-temporal cache policy (NT): for streams read once per
+// kernel whose lines are not reused from the caches before eviction (history
+// levels of the march).  Measured on the Westervelt step: 83.4 -> 76.4 us at
+// 192^3 (340 MB state), but slower at 128^3, whose 100 MB state the 256 MB
+// last-level cache (MALL) holds between steps.
+constexpr size_t kMallBytes = 256ull << 20;
+constexpr int kCachePolicyNT = 2;
+template <typename T>
+__device__ __forceinline__ T bload_nt(rsrc_t r, uint32_t voff) {
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, 0, kCachePolicyNT));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, 0, kCachePolicyNT));
+}
+
 template <typename T>
 __device__ __forceinline__ void bstore(T v, rsrc_t r, uint32_t voff);
 template <>

@@ -389,8 +389,12 @@ static WvSrc make_src(const bd_westervelt_params* p, const double* amp, const do
 template <typename T, int V>
 static void launch_variant(const WvMarchArgs<T>& a, int step_arg, hipStream_t st) {
     constexpr WvVariant v = kWvVariants[V];
-    hipLaunchKernelGGL((wv_march<T, v.tx, v.ty, v.ry, (bool)v.pf>), dim3(a.g.nblocks), dim3(v.tx * v.ty), 0,
-                       st, a, step_arg);
+    if (a.nt)
+        hipLaunchKernelGGL((wv_march<T, v.tx, v.ty, v.ry, (bool)v.pf, true>), dim3(a.g.nblocks),
+                           dim3(v.tx * v.ty), 0, st, a, step_arg);
+    else
+        hipLaunchKernelGGL((wv_march<T, v.tx, v.ty, v.ry, (bool)v.pf, false>), dim3(a.g.nblocks),
+                           dim3(v.tx * v.ty), 0, st, a, step_arg);
 }
 
 template <typename T>
@@ -445,7 +449,8 @@ static int launch_zfaces(const WvFaceArgs<T>& f, hipStream_t st) {
     return BD_OK;
 }
 
-// Fills the march arguments that fuse the source plane and the z, x and y faces.
+// Fills the march arguments that fuse the source plane and the z, x and y
+// faces and choose the history-load cache policy (a.g must be set).
 template <typename T>
 static void set_fused_z(WvMarchArgs<T>& a, const bd_westervelt_params* p, int on, const double* amp,
                         const double* phase) {
@@ -454,6 +459,8 @@ static void set_fused_z(WvMarchArgs<T>& a, const bd_westervelt_params* p, int on
     a.kz_w = p->c_inf * p->dt / p->dz;
     a.kz_t = p->c_inf_tissue * p->dt / p->dz;
     a.fuse_xy = on;
+    // non-temporal history loads once the 12-field state outgrows the MALL
+    a.nt = (size_t)12 * a.g.Nxy * a.g.Nxy * a.g.Nz * sizeof(T) > kMallBytes ? 1 : 0;
     a.kxy_w = p->c_inf * p->dt / p->dxy;
     a.kxy_t = p->c_inf_tissue * p->dt / p->dxy;
     a.d_time = nullptr;

@@ -7,7 +7,9 @@
 //     buffered: one barrier per plane);
 //   * z-neighbours come from a 7-deep register queue per column fed by one
 //     coalesced load of plane z+3;
-//   * p^{n-1..n-4}, R1/R2 history and pmax are streamed once;
+//   * p^{n-1..n-4}, R1/R2 history and pmax are streamed once (the history
+//     levels with the non-temporal cache policy when the state is larger
+//     than the 256 MB last-level cache, which otherwise keeps it);
 //   * with PF, all loads of plane z+1 are issued before plane z is computed.
 // Fused outputs: pmax, per-block energy partials, the focus sample; with
 // fuse_z the source plane and the z faces; with fuse_xy the x/y Mur faces,
@@ -104,6 +106,7 @@ struct WvMarchArgs {
     int fuse_z, boundary;
     double kz_w, kz_t;         // Courant numbers of the z faces (W:100, W:111)
     int fuse_xy;               // x/y faces after the march (requires fuse_z)
+    int nt;                    // launch the instantiation with non-temporal history loads
     double kxy_w, kxy_t;       // Courant numbers of the x/y faces (W:117-119)
     const double* d_time;      // time table (graph replay), t = d_time[step+1]
     double t_next;             // t^{n+1} when d_time is null
@@ -135,7 +138,7 @@ struct PlaneIn {
     T r1m1[RY], r1m2[RY], r2m1[RY], r2m2[RY], pmx[RY];
 };
 
-template <typename T, int TX, int TY, int RY, bool PF>
+template <typename T, int TX, int TY, int RY, bool PF, bool NTL>
 __global__ void __launch_bounds__(TX * TY)
 wv_march(WvMarchArgs<T> a, int step_arg) {
     constexpr int NT = TX * TY;
@@ -200,8 +203,13 @@ wv_march(WvMarchArgs<T> a, int step_arg) {
             v.qn[r] = bload<T>(rp, o, 3 * pbytes);
             v.hl[r] = xL ? bload<T>(rp, o + (uint32_t)(dxl * (int)ES)) : T(0);
             v.hr[r] = xR ? bload<T>(rp, o + 3 * ES) : T(0);
-            v.pm1[r] = bload<T>(q1, o); v.pm2[r] = bload<T>(q2, o);
-            v.pm3[r] = bload<T>(q3, o); v.pm4[r] = bload<T>(q4, o);
+            if constexpr (NTL) {
+                v.pm1[r] = bload_nt<T>(q1, o); v.pm2[r] = bload_nt<T>(q2, o);
+                v.pm3[r] = bload_nt<T>(q3, o); v.pm4[r] = bload_nt<T>(q4, o);
+            } else {
+                v.pm1[r] = bload<T>(q1, o); v.pm2[r] = bload<T>(q2, o);
+                v.pm3[r] = bload<T>(q3, o); v.pm4[r] = bload<T>(q4, o);
+            }
         }
         if (tis) {
             const rsrc_t s1 = make_rsrc(a.r1m1 + zo, pbytes), s2 = make_rsrc(a.r1m2 + zo, pbytes);
@@ -209,8 +217,13 @@ wv_march(WvMarchArgs<T> a, int step_arg) {
 #pragma unroll
             for (int r = 0; r < RY; ++r) {
                 const uint32_t o = cidx[r] * ES;
-                v.r1m1[r] = bload<T>(s1, o); v.r1m2[r] = bload<T>(s2, o);
-                v.r2m1[r] = bload<T>(s3, o); v.r2m2[r] = bload<T>(s4, o);
+                if constexpr (NTL) {
+                    v.r1m1[r] = bload_nt<T>(s1, o); v.r1m2[r] = bload_nt<T>(s2, o);
+                    v.r2m1[r] = bload_nt<T>(s3, o); v.r2m2[r] = bload_nt<T>(s4, o);
+                } else {
+                    v.r1m1[r] = bload<T>(s1, o); v.r1m2[r] = bload<T>(s2, o);
+                    v.r2m1[r] = bload<T>(s3, o); v.r2m2[r] = bload<T>(s4, o);
+                }
             }
         } else {
 #pragma unroll
