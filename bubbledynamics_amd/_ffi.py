@@ -123,6 +123,7 @@ BD_GM_SCHED_SLICED = 1
 _VP = C.c_void_p
 _I = C.c_int
 _D = C.c_double
+_IP = C.POINTER(C.c_int)
 _PP = C.POINTER(WesterveltParams)
 
 # name -> (restype, argtypes); entries added by the other modules' binding tables
@@ -147,6 +148,7 @@ SIGNATURES: dict[str, tuple] = {
     "bd_westervelt_plan_destroy": (_I, [_VP]),
     "bd_westervelt_plan_rerun_step": (_I, [_VP, _VP]),
     "bd_westervelt_plan_interior_cells": (C.c_longlong, [_VP]),
+    "bd_westervelt_plan_shape": (_I, [_VP, _IP, _IP, _IP, _IP, _IP, _IP]),
     "bd_westervelt_adjoint": (_I, [_PP, _I, _I, _I, _I, _I, _VP, _VP, C.POINTER(AdjointObjective),
                                    C.POINTER(_D), _VP, _VP, C.POINTER(_D), _VP]),
     "bd_temperature_square": (_I, [C.POINTER(BioheatParams), _I, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _I, _VP]),

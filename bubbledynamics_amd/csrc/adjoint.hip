@@ -78,6 +78,7 @@ struct AdjPlane {
 // the sponge factor is formed once per cell when lam^n is written.
 template <typename T, int kAX, int kAY>
 __global__ void __launch_bounds__(kAX * kAY) adj_step_kernel(AdjArgs<T> a) {
+    static_assert(kAY >= 3 && kAX >= 6, "tile too small for the 3-cell halo");
     __shared__ T s[2][kAY + 6][kAX + 6];
     const int lb = xcd_remap((int)blockIdx.x, a.nblocks);
     const int zc = lb % a.nzc, t2 = lb / a.nzc;
@@ -251,7 +252,7 @@ static int adjoint_t(const bd_westervelt_params* prm, int Nxy, int Nz, int Nt, i
     const size_t plane = (size_t)N * N, NF = plane * Nz, FB = NF * sizeof(T);
     if (K <= 0 || K > Nt) K = Nt;
     const int nseg = (Nt + K - 1) / K;
-    const int nb = wv_forward_nblocks(prm, Nxy, Nz);
+    const int nb = wv_forward_nblocks(prm, sizeof(T) == 8 ? BD_F64 : BD_F32, Nxy, Nz);
     // forward workspace: segment store p^{a-4..a+K}, R rings, checkpoints
     DevBuf seg[2], rring, ckp, epart, jsum;
     if (!seg[0].alloc(FB * (size_t)(K + 5)) || !rring.alloc(FB * 6) || !epart.alloc(sizeof(double) * 2 * nb * (size_t)Nt) ||

@@ -306,29 +306,14 @@ __global__ void energy_finalize_kernel(const double* epart, double* energy, int 
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
-constexpr int kDefaultVariant = 0;
-constexpr int kWide128Variant = 8;
+constexpr int kDefaultVariant = 0;   // 64 x 4 tile, one row per thread
+constexpr int kDeepVariant = 2;      // 64 x 4 tile, two rows per thread, plane prefetch
 constexpr int kDefaultBlocks = 1024;
 
-// Variant / target block count for the interior kernel; BD_WV_VARIANT and
-// BD_WV_BLOCKS override the defaults (used by the tile-shape sweep).  The
-// default tile is 64 or 128 columns wide, whichever leaves fewer idle lanes
-// across the inside columns (the wider one on a tie: fewer halo columns).
-static void launch_shape(int* variant, int* blocks, int nin = 0) {
-    *variant = kDefaultVariant;
-    if (nin > 0) {
-        const int w64 = (nin + 63) / 64 * 64 - nin, w128 = (nin + 127) / 128 * 128 - nin;
-        if (w128 <= w64) *variant = kWide128Variant;
-    }
-    *blocks = kDefaultBlocks;
-    if (const char* v = std::getenv("BD_WV_VARIANT")) {
-        const int x = std::atoi(v);
-        if (x >= 0 && x < kWvNumVariants) *variant = x;
-    }
-    if (const char* v = std::getenv("BD_WV_BLOCKS")) {
-        const int x = std::atoi(v);
-        if (x > 0) *blocks = x;
-    }
+// The 12-field state does not fit in the last-level cache: the step kernel
+// then loads the history levels with the non-temporal policy.
+static bool streams_past_mall(int Nxy, int Nz, size_t elem_bytes) {
+    return (size_t)12 * Nxy * Nxy * Nz * elem_bytes > kMallBytes;
 }
 
 static int check_params(const bd_westervelt_params* p, int dtype, int Nxy, int Nz) {
@@ -372,6 +357,31 @@ static WvGeom make_geom(const bd_westervelt_params* p, int Nxy, int Nz, int vari
     return g;
 }
 
+// Variant / target block count for the interior kernel; BD_WV_VARIANT and
+// BD_WV_BLOCKS override the defaults (used by the tile-shape sweep).  FP32
+// takes the two-row prefetching tile unless its z-chunks (at least 8 planes)
+// leave it fewer than 3/4 of the one-row tile's workgroups, as on small grids;
+// FP64, whose two-row tile holds twice the registers, takes the one-row tile.
+// FP32 steps measured on one MI355X (quarter domain, us, one-row / two-row):
+// 128^3 26.7 / 30.4 (512 vs 1024 workgroups), 160^3 47.6 / 46.7,
+// 192^3 76.3 / 74.0, 208^3 135.8 / 116.3, 224^3 144.0 / 120.9,
+// 256^3 204.0 / 188.0, 320^3 424.3 / 410.2; FP64 192^3 171.3 / 176.3.
+static void launch_shape(const bd_westervelt_params* p, int Nxy, int Nz, bool f64, int* variant, int* blocks) {
+    *variant = kDefaultVariant;
+    *blocks = kDefaultBlocks;
+    if (!f64 && 4 * make_geom(p, Nxy, Nz, kDeepVariant, kDefaultBlocks).nblocks >=
+                    3 * make_geom(p, Nxy, Nz, kDefaultVariant, kDefaultBlocks).nblocks)
+        *variant = kDeepVariant;
+    if (const char* v = std::getenv("BD_WV_VARIANT")) {
+        const int x = std::atoi(v);
+        if (x >= 0 && x < kWvNumVariants) *variant = x;
+    }
+    if (const char* v = std::getenv("BD_WV_BLOCKS")) {
+        const int x = std::atoi(v);
+        if (x > 0) *blocks = x;
+    }
+}
+
 static WvSrc make_src(const bd_westervelt_params* p, const double* amp, const double* phase) {
     WvSrc s;
     s.kind = p->source;
@@ -409,7 +419,6 @@ static int launch_march(const WvMarchArgs<T>& a, int variant, int step_arg, hipS
         case 6: launch_variant<T, 6>(a, step_arg, st); break;
         case 7: launch_variant<T, 7>(a, step_arg, st); break;
         case 8: launch_variant<T, 8>(a, step_arg, st); break;
-        case 9: launch_variant<T, 9>(a, step_arg, st); break;
         default: return BD_ERR_INVALID;
     }
     BD_LAUNCH_CHECK();
@@ -460,7 +469,7 @@ static void set_fused_z(WvMarchArgs<T>& a, const bd_westervelt_params* p, int on
     a.kz_t = p->c_inf_tissue * p->dt / p->dz;
     a.fuse_xy = on;
     // non-temporal history loads once the 12-field state outgrows the MALL
-    a.nt = (size_t)12 * a.g.Nxy * a.g.Nxy * a.g.Nz * sizeof(T) > kMallBytes ? 1 : 0;
+    a.nt = streams_past_mall(a.g.Nxy, a.g.Nz, sizeof(T)) ? 1 : 0;
     a.kxy_w = p->c_inf * p->dt / p->dxy;
     a.kxy_t = p->c_inf_tissue * p->dt / p->dxy;
     a.d_time = nullptr;
@@ -479,7 +488,7 @@ static int forward_step_t(const bd_westervelt_params* prm, int Nxy, int Nz, cons
     a.r2 = (T*)q.r2; a.r2m1 = (const T*)q.r2m1; a.r2m2 = (const T*)q.r2m2;
     a.pmax = nullptr; a.epart = epart; a.focus = nullptr; a.d_step = nullptr; a.d_step_next = nullptr; a.focus_idx = -1;
     int variant, blocks;
-    launch_shape(&variant, &blocks, Nxy - 3 - (prm->full_domain ? 3 : 0));
+    launch_shape(prm, Nxy, Nz, sizeof(T) == 8, &variant, &blocks);
     a.g = make_geom(prm, Nxy, Nz, variant, blocks);
     if (obj) { a.g.obj_on = 1; a.g.ocx = obj->cx; a.g.ocy = obj->cy; a.g.ocz = obj->cz; a.g.or2 = obj->r2; }
     a.c = make_coef<T>(prm);
@@ -495,9 +504,9 @@ int wv_forward_step(const bd_westervelt_params* prm, int dtype, int Nxy, int Nz,
     return forward_step_t<double>(prm, Nxy, Nz, q, step, t_next, amp, phase, epart, obj, st);
 }
 
-int wv_forward_nblocks(const bd_westervelt_params* prm, int Nxy, int Nz) {
+int wv_forward_nblocks(const bd_westervelt_params* prm, int dtype, int Nxy, int Nz) {
     int variant, blocks;
-    launch_shape(&variant, &blocks, Nxy - 3 - (prm->full_domain ? 3 : 0));
+    launch_shape(prm, Nxy, Nz, dtype == BD_F64, &variant, &blocks);
     return make_geom(prm, Nxy, Nz, variant, blocks).nblocks;
 }
 
@@ -524,7 +533,7 @@ static int space6_t(const bd_westervelt_params* prm, void* Pnext, const void* P,
     a.r2 = (T*)R2; a.r2m1 = (const T*)R2m1; a.r2m2 = (const T*)R2m2;
     a.pmax = nullptr; a.epart = nullptr; a.focus = nullptr; a.d_step = nullptr; a.d_step_next = nullptr; a.focus_idx = -1;
     int variant, blocks;
-    launch_shape(&variant, &blocks, Nxy - 3 - (prm->full_domain ? 3 : 0));
+    launch_shape(prm, Nxy, Nz, sizeof(T) == 8, &variant, &blocks);
     a.g = make_geom(prm, Nxy, Nz, variant, blocks);
     a.c = make_coef<T>(prm);
     set_fused_z<T>(a, prm, 0, nullptr, nullptr);
@@ -691,7 +700,7 @@ extern "C" int bd_westervelt_plan_create(const bd_westervelt_params* prm, int dt
     auto* pl = new bd_westervelt_plan();
     pl->prm = *prm; pl->dtype = dtype; pl->Nxy = Nxy; pl->Nz = Nz; pl->max_steps = max_steps;
     int blocks = 0;
-    launch_shape(&pl->variant, &blocks, Nxy - 3 - (prm->full_domain ? 3 : 0));
+    launch_shape(prm, Nxy, Nz, dtype == BD_F64, &pl->variant, &blocks);
     pl->g = make_geom(prm, Nxy, Nz, pl->variant, blocks);
     const int lo = prm->full_domain ? 3 : 0;
     pl->focus_idx = -1;
@@ -853,6 +862,20 @@ extern "C" long long bd_westervelt_plan_interior_cells(const bd_westervelt_plan*
     if (!pl) return -1;
     const long long nin = pl->Nxy - 3 - (pl->prm.full_domain ? 3 : 0);
     return nin * nin * (long long)(pl->Nz - 6);
+}
+
+extern "C" int bd_westervelt_plan_shape(const bd_westervelt_plan* pl, int* tile_x, int* tile_y, int* rows,
+                                        int* prefetch, int* nontemporal, int* workgroups) {
+    if (!pl) return BD_ERR_INVALID;
+    const WvVariant v = kWvVariants[pl->variant];
+    const size_t es = pl->dtype == BD_F64 ? 8 : 4;
+    if (tile_x) *tile_x = v.tx;
+    if (tile_y) *tile_y = v.ty;
+    if (rows) *rows = v.ry;
+    if (prefetch) *prefetch = v.pf;
+    if (nontemporal) *nontemporal = streams_past_mall(pl->Nxy, pl->Nz, es) ? 1 : 0;
+    if (workgroups) *workgroups = pl->g.nblocks;
+    return BD_OK;
 }
 
 extern "C" int bd_westervelt_plan_finalize(bd_westervelt_plan* pl, void* stream) {

@@ -24,7 +24,7 @@ struct WvObjective {
 int wv_forward_step(const bd_westervelt_params* prm, int dtype, int Nxy, int Nz, const WvStepPtrs& q, int step,
                     double t_next, const double* amp, const double* phase, double* epart, const WvObjective* obj,
                     hipStream_t st);
-int wv_forward_nblocks(const bd_westervelt_params* prm, int Nxy, int Nz);
+int wv_forward_nblocks(const bd_westervelt_params* prm, int dtype, int Nxy, int Nz);
 int wv_check_params(const bd_westervelt_params* p, int dtype, int Nxy, int Nz);
 
 }  // namespace bd

@@ -144,6 +144,8 @@ wv_march(WvMarchArgs<T> a, int step_arg) {
     constexpr int NT = TX * TY;
     constexpr int ROWS = TY * RY;
     constexpr bool F32 = std::is_same<T, float>::value;
+    // the three y-halo rows on each side are loaded one per thread row
+    static_assert(TY >= 3 && TX >= 6, "tile too small for the 3-cell halo");
     __shared__ T s[2][ROWS + 6][TX + 6];
     __shared__ double red[NT / kWave];
 
@@ -563,7 +565,7 @@ struct WvVariant { int tx, ty, ry, pf; };
 constexpr WvVariant kWvVariants[] = {
     {64, 4, 1, 0}, {64, 4, 1, 1}, {64, 4, 2, 1}, {64, 8, 1, 1},
     {64, 8, 2, 1}, {64, 4, 4, 1}, {64, 8, 2, 0}, {32, 8, 2, 1},
-    {128, 4, 1, 0}, {128, 2, 2, 0},
+    {128, 4, 1, 0},
 };
 constexpr int kWvNumVariants = sizeof(kWvVariants) / sizeof(kWvVariants[0]);
 

@@ -78,6 +78,15 @@ class WesterveltRun:
     def steps(self) -> int:
         return int(_ffi.load().bd_westervelt_plan_steps(self._plan))
 
+    @property
+    def shape(self) -> dict:
+        """Launch shape of the step kernel (tile, rows per thread, prefetch,
+        non-temporal history loads, workgroups per step)."""
+        v = [C.c_int() for _ in range(6)]
+        call("bd_westervelt_plan_shape", self._plan, *[C.byref(x) for x in v])
+        keys = ("tile_x", "tile_y", "rows", "prefetch", "nontemporal", "workgroups")
+        return {k: x.value for k, x in zip(keys, v)}
+
     def seek(self, step: int) -> None:
         """The state block holds the levels of `step` (e.g. a restored copy)."""
         call("bd_westervelt_plan_seek", self._plan, int(step))

@@ -77,3 +77,19 @@ def test_fp32_adjoint_close_to_fp64():
     print(f"FP32 vs FP64: J {abs(J32 / J64 - 1):.2e}, dA {eA:.2e}, dphi {eP:.2e}")
     assert abs(J32 / J64 - 1) < 1e-4
     assert eA < 1e-3 and eP < 1e-3
+
+
+@pytest.mark.parametrize("tile", range(5))
+def test_every_adjoint_tile_same_gradient(monkeypatch, tile):
+    # the backward kernel's tile shapes (64x4, 64x8, 32x8, 128x4, 128x8,
+    # selected with BD_ADJ_TILE) give the default tile's gradient up to the
+    # summation order of the per-workgroup partials
+    n, steps = 40, 80
+    p, amp, ph, obj = setup(n, steps)
+    monkeypatch.delenv("BD_ADJ_TILE", raising=False)
+    J0, gA0, gP0, _ = adjoint_gradient(p, n, steps, amp, ph, obj, 20, torch.float64)
+    monkeypatch.setenv("BD_ADJ_TILE", str(tile))
+    J1, gA1, gP1, _ = adjoint_gradient(p, n, steps, amp, ph, obj, 20, torch.float64)
+    assert J1 == J0
+    assert np.abs(gA1 - gA0).max() <= 1e-12 * np.abs(gA0).max()
+    assert np.abs(gP1 - gP0).max() <= 1e-12 * np.abs(gP0).max()

@@ -231,6 +231,99 @@ def test_plan_faces_at_tile_edges(orc, n, full):
     assert np.abs(e[:, 0] - en[:, 0]).max() <= 1e-11 * np.abs(en[:, 0]).max()
 
 
+# tile shapes the step kernel is instantiated for: (columns, thread rows,
+# rows per thread, plane prefetch), selected with BD_WV_VARIANT
+TILE_SHAPES = [(64, 4, 1, 0), (64, 4, 1, 1), (64, 4, 2, 1), (64, 8, 1, 1), (64, 8, 2, 1),
+               (64, 4, 4, 1), (64, 8, 2, 0), (32, 8, 2, 1), (128, 4, 1, 0)]
+_TILE_ORACLE = {}
+
+
+def _tile_case(orc, n, full, nz=24, steps=60):
+    key = (n, full, nz, steps)
+    if key not in _TILE_ORACLE:
+        p = presets.reference_scaled(n, nz)
+        p.d = 0.8 * (n - 4) * p.dxy
+        p.alpha, p.alpha_hole = math.asin(0.9), 0.0
+        if full:
+            p.full_domain = 1
+            p.xy_min = -(n - 1) / 2 * p.dxy
+        focus = (n // 2, n // 2, nz // 2)
+        _TILE_ORACLE[key] = (p, focus, _oracle_run(orc, p, n, nz, steps, focus) if orc else None)
+    return _TILE_ORACLE[key]
+
+
+def _run_shape(monkeypatch, variant, p, n, nz, steps, focus, dtype):
+    if variant is None:
+        monkeypatch.delenv("BD_WV_VARIANT", raising=False)
+    else:
+        monkeypatch.setenv("BD_WV_VARIANT", str(variant))
+    run = WesterveltRun(p, n, nz, dtype=dtype, max_steps=steps, focus=focus)
+    monkeypatch.delenv("BD_WV_VARIANT", raising=False)
+    sh = run.shape
+    if variant is not None:
+        assert (sh["tile_x"], sh["tile_y"], sh["rows"], sh["prefetch"]) == TILE_SHAPES[variant]
+    run.run(steps, use_graph=True)
+    run.finalize()
+    torch.cuda.synchronize()
+    return run
+
+
+@pytest.mark.parametrize("variant", range(len(TILE_SHAPES)))
+@pytest.mark.parametrize("n,full", [(67, 0), (70, 1)])
+def test_every_tile_shape_matches_oracle(orc, monkeypatch, variant, n, full):
+    # every instantiated tile shape (not only the default one) against the FP64
+    # oracle on grids whose last inside row and column end a 64 x 4 tile
+    nz, steps = 24, 60
+    p, focus, (st, pmax, en, fs) = _tile_case(orc, n, full, nz, steps)
+    run = _run_shape(monkeypatch, variant, p, n, nz, steps, focus, torch.float64)
+    ref = st[steps % 6]
+    assert np.abs(host(run.pressure(0)) - ref).max() <= 1e-11 * np.abs(ref).max()
+    assert np.abs(host(run.pmax) - pmax).max() <= 1e-11 * np.abs(pmax).max()
+    e = host(run.energy)
+    assert np.abs(e[:, 0] - en[:, 0]).max() <= 1e-11 * np.abs(en[:, 0]).max()
+    assert np.abs(host(run.focus) - fs).max() <= 1e-11 * np.abs(fs).max()
+
+
+@pytest.mark.parametrize("variant", range(1, len(TILE_SHAPES)))
+def test_tile_shapes_fp32_bitwise_equal(monkeypatch, variant):
+    # the per-cell arithmetic does not depend on the tile: FP32 fields, pmax and
+    # focus record are bit-identical across shapes; the per-step energies sum
+    # per-thread Kahan-compensated FP32 partials (each within 2 u = 1.2e-7 of
+    # its exact sum) whose grouping follows the tile
+    n, nz, steps = 67, 40, 120
+    p, focus, _ = _tile_case(None, n, 0, nz, steps)
+    a = _run_shape(monkeypatch, 0, p, n, nz, steps, focus, torch.float32)
+    b = _run_shape(monkeypatch, variant, p, n, nz, steps, focus, torch.float32)
+    assert np.abs(host(a.pressure(0))).max() > 0
+    for lvl in range(5):
+        assert torch.equal(a.pressure(lvl), b.pressure(lvl))
+    for which in (1, 2):
+        assert torch.equal(a.relax(which, 0), b.relax(which, 0))
+    assert torch.equal(a.pmax, b.pmax)
+    assert torch.equal(a.focus, b.focus)
+    ea, eb = host(a.energy), host(b.energy)
+    for k in range(2):
+        assert np.abs(ea[:, k] - eb[:, k]).max() <= 1e-6 * np.abs(ea[:, k]).max()
+
+
+def test_default_shape_rule(monkeypatch):
+    # FP32 takes the two-row prefetching tile unless it would leave fewer than
+    # 3/4 of the one-row tile's workgroups; FP64 keeps the one-row tile; the
+    # history loads are non-temporal once the 12-field state exceeds 256 MB
+    monkeypatch.delenv("BD_WV_VARIANT", raising=False)
+    monkeypatch.delenv("BD_WV_BLOCKS", raising=False)
+    expect = {(192, torch.float32): (2, 1, 1), (128, torch.float32): (1, 0, 0),
+              (160, torch.float32): (2, 1, 0), (192, torch.float64): (1, 0, 1)}
+    for (n, dt), (rows, pf, nt) in expect.items():
+        p = presets.tone_burst_192(n)
+        run = WesterveltRun(p, n, n, dtype=dt, max_steps=1, pmax=False, energy=False)
+        sh = run.shape
+        assert (sh["rows"], sh["prefetch"], sh["nontemporal"]) == (rows, pf, nt), (n, dt, sh)
+        assert sh["tile_x"] == 64 and 768 <= sh["workgroups"] <= 1024
+        run.close()
+        del run
+
+
 def test_tone_burst_small_fp32_vs_fp64(orc):
     # the benchmark workload's physics at 64^3: FP32 focal pressure against the
     # FP64 oracle, relative L2 over the whole record
