@@ -7,8 +7,18 @@ peak pressure, per-step energy and the focus sample) -- one kernel launch,
 replayed from a pair of captured 6-step hipGraphs.  Inputs live in HBM
 before timing starts.
 
-Prints one JSON line.  `--gpus N` with N > 1 exits: this build measures one
-GPU (the z-slab sharding across GPUs is described in the design notes).
+Prints one JSON line.  `--gpus N` with N > 1 exits: one GPU is measured (the
+z-slab decomposition across GPUs is not built).
+
+`--reference-scenario` instead runs the reference's own programs at their own
+scale and prints one JSON line of stage timings: the heat source of
+Westervelt_relax_fft_Pressure_Energy.cu (426 x 426 x 2003 quarter domain,
+FP64, 165 settle periods, one water period, 4 harmonic periods, 10 harmonics),
+its Q3D file, the quarter -> full expansion of
+Westervelt_Temperature_SquareSpiral.cu (851 x 851 x 2001) and that program's
+continuous spiral (all 77,474 substeps, or `--shifts`) in FP32 with the fields
+stored as T - T0 and a compensated dose; the first `--compare` substeps also
+run in FP64 (the reference's precision) to state the FP32 error.
 """
 from __future__ import annotations
 
@@ -169,6 +179,152 @@ def coupled_figures() -> dict:
             "max_cem43_min": float(run.dose().max().item() / 60.0)}
 
 
+def reference_scale(shifts: int, compare: int = -1, out_dir: str | None = None) -> dict:
+    """The reference's heat-source and thermal programs at their own grids."""
+    import ctypes as C
+    import tempfile
+    import torch
+    from bubbledynamics_amd import _ffi, bioheat as bh, workloads as wl
+    from bubbledynamics_amd._ffi import BD_F64
+    out = {}
+    t0 = time.perf_counter()
+    stamps = {}
+
+    def progress(stage, step):
+        now = time.perf_counter() - t0
+        stamps[stage] = now
+        log(f"{stage} {step} steps, {now:.1f} s")
+
+    Q, p, info = wl.reference_scenario(full_scale=True, settle_periods=165, periods=4, n_harm=10,
+                                       progress=progress)
+    nz, nxy = Q.shape[0], Q.shape[1]
+    steps = info["settle"] + info["water"] + info["harmonics"]
+    cells = nxy * nxy * nz
+    t_w = time.perf_counter() - t0
+    iz_t = info["iz_tissue"]
+    qt = Q[iz_t:]
+    k = int(torch.argmax(qt).item())
+    out["westervelt"] = {
+        "grid": [nxy, nxy, nz], "dtype": "f64", "steps": steps, "seconds": round(t_w, 2),
+        "settle_s": round(stamps["settle"], 2),
+        "mcell_steps_per_s": round(cells * steps / t_w / 1e6, 1),
+        "q_tissue_max_w_m3": float(qt.max().item()),
+        "q_tissue_argmax_izyx": [iz_t + k // (nxy * nxy), (k // nxy) % nxy, k % nxy],
+        "q_water_max_w_m3": float(Q[:iz_t].max().item()),
+        "finite": bool(torch.isfinite(Q).all().item())}
+    log(f"westervelt: {out['westervelt']}")
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "Q3D_reference.bin")
+        t1 = time.perf_counter()
+        _ffi.call("bd_q3d_write", path.encode(), BD_F64, C.c_void_p(Q.data_ptr()), nxy, nz, 2, st)
+        t_write = time.perf_counter() - t1
+        size = os.path.getsize(path)
+        del Q, qt
+        torch.cuda.empty_cache()
+        bp, nb, nzb = bh.reference_params()
+        assert nb == 2 * (nxy - 1) + 1 and nzb == nz - 2
+        Qf = torch.empty((nzb, nb, nb), dtype=torch.float64, device="cuda")
+        t1 = time.perf_counter()
+        _ffi.call("bd_q3d_read_expand", path.encode(), BD_F64, C.c_void_p(Qf.data_ptr()), nb, nzb, st)
+        t_read = time.perf_counter() - t1
+    out["q3d"] = {"bytes": size, "write_s": round(t_write, 2), "read_expand_s": round(t_read, 2)}
+    log(f"q3d: {out['q3d']}")
+    sx, sy = bh.spiral_reference()
+    n = len(sx) if shifts <= 0 else min(shifts, len(sx))
+
+    def runs_of(lo, hi):
+        # consecutive equal shifts run as one library call
+        out_runs, i = [], lo
+        while i < hi:
+            j = i
+            while j + 1 < hi and sx[j + 1] == sx[i] and sy[j + 1] == sy[i]:
+                j += 1
+            out_runs.append((int(sx[i]), int(sy[i]), j - i + 1))
+            i = j + 1
+        return out_runs
+
+    def advance(run, lo, hi, label):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        done, mark = lo, lo
+        for a_, b_, m in runs_of(lo, hi):
+            run.step(a_, b_, m)
+            done += m
+            if done - mark >= 4000:
+                mark = done
+                torch.cuda.synchronize()
+                log(f"{label}: {done}/{hi} substeps, {time.perf_counter() - t1:.1f} s")
+        torch.cuda.synchronize()
+        return time.perf_counter() - t1
+
+    bcells = (nb - 2) * (nb - 2) * (nzb - 2)
+    T0_tissue = 23.6 + 273.15                                  # T:149
+    ends = bh.spiral_reference_line_ends()
+    assert ends[-1] == len(sx)
+    # FP64 drop-in precision on the first `compare` substeps (default: the first spiral line)
+    m_cmp = min(ends[0] if compare < 0 else compare, n)
+    ref64 = None
+    if m_cmp > 0:
+        run64 = bh.BioheatRun(bp, nb, nzb, Qf, 0.0, dtype=torch.float64, kahan=False, offset=False)
+        run64.set_reference_initial()
+        t64 = advance(run64, 0, m_cmp, "FP64")
+        ref64 = (run64.temperature(), run64.dose())
+        out["bioheat_fp64"] = {"substeps": m_cmp, "seconds": round(t64, 2),
+                               "mcell_steps_per_s": round(bcells * m_cmp / t64 / 1e6, 1)}
+        log(f"bioheat_fp64: {out['bioheat_fp64']}")
+        del run64
+        torch.cuda.empty_cache()
+    # FP32 with fields stored as T - T0 and a compensated dose, all substeps
+    p32 = bp.copy()
+    p32.T_art = T0_tissue                      # the storage offset (no perfusion in this program)
+    run = bh.BioheatRun(p32, nb, nzb, Qf, 0.0, dtype=torch.float32, kahan=True, offset=True)
+    del Qf
+    torch.cuda.empty_cache()
+    run.set_reference_initial()
+    t_b = advance(run, 0, m_cmp, "FP32")
+    if ref64 is not None:
+        T64, D64 = ref64
+        dT = (run.temperature() - T64).abs().max().item()
+        sel = D64 > 1e-3 * D64.max()
+        drel = ((run.dose()[sel] / D64[sel]) - 1).abs().max().item()
+        out["bioheat_fp64"].update({"fp32_max_abs_dT_k": dT, "fp32_rise_k": float((T64.max() - T0_tissue).item()),
+                                    "fp32_dose_max_rel_err": drel})
+        log(f"FP32 vs FP64 after {m_cmp} substeps: max |dT| {dT:.2e} K, dose rel err {drel:.2e}")
+        del T64, D64, ref64, sel
+        torch.cuda.empty_cache()
+    if out_dir:
+        os.makedirs(out_dir, exist_ok=True)
+    from bubbledynamics_amd._ffi import BD_SLICE_XY, BD_SLICE_XZ
+    # the reference's T slices after spiral lines 1, 5, 9, 13 (T:411-563)
+    done = m_cmp
+    for k, line in enumerate((1, 5, 9, 13)):
+        stop = min(ends[line - 1], n)
+        if stop > done:
+            t_b += advance(run, done, stop, "FP32")
+            done = stop
+        if out_dir and done == ends[line - 1]:
+            run.write_slice(os.path.join(out_dir, f"T_XZ_loop{k}.txt"), BD_SLICE_XZ, 425, 1200, 1701, 325, 526)
+            run.write_slice(os.path.join(out_dir, f"T_XY_loop{k}_z15.txt"), BD_SLICE_XY, 1500, 325, 526, 325, 526)
+    if n > done:
+        t_b += advance(run, done, n, "FP32")
+    T = run.temperature()
+    out["bioheat"] = {"grid": [nb, nb, nzb], "dtype": "f32 (T - T0, Kahan dose)", "substeps": n,
+                      "of_reference": len(sx), "calls": len(runs_of(0, n)), "seconds": round(t_b, 2),
+                      "mcell_steps_per_s": round(bcells * n / t_b / 1e6, 1),
+                      "t_max_c": float(T.max().item() - 273.15),
+                      "dose_max_min": float(run.dose().max().item() / 60.0),
+                      "finite": bool(torch.isfinite(T).all().item())}
+    if out_dir:
+        # the reference's end-of-run dose slices (T:573-634)
+        run.write_slice(os.path.join(out_dir, "TD_XZ.txt"), BD_SLICE_XZ, 425, 1200, 1601, 325, 527, "TD")
+        for iz, tag in ((1500, "z15"), (1350, "z13_5"), (1250, "z12_5")):
+            run.write_slice(os.path.join(out_dir, f"TD_XY_{tag}.txt"), BD_SLICE_XY, iz, 325, 526, 325, 527, "TD")
+        out["bioheat"]["slices"] = out_dir
+    log(f"bioheat: {out['bioheat']}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,10 +338,19 @@ def main() -> int:
                     help="skip the Gilmore, adjoint and coupled bioheat figures")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first copy a buffer of this size 3 times (HBM counter calibration)")
+    ap.add_argument("--reference-scenario", action="store_true",
+                    help="run the reference's programs at their own scale instead (one JSON line)")
+    ap.add_argument("--shifts", type=int, default=0,
+                    help="spiral substeps of the thermal program in --reference-scenario (0: all 77,474)")
+    ap.add_argument("--compare", type=int, default=-1,
+                    help="substeps also run in FP64 to state the FP32 error in --reference-scenario "
+                         "(-1: the first spiral line, 1,802)")
+    ap.add_argument("--slices", default="",
+                    help="directory for the reference's end-of-run slice files in --reference-scenario")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: the multi-GPU z-slab decomposition is not built "
-              "in this version (see the design notes); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the multi-GPU z-slab decomposition is not built; "
+              "run with --gpus 1", file=sys.stderr)
         return 2
 
     import numpy as np
@@ -197,6 +362,10 @@ def main() -> int:
         print("no GPU visible", file=sys.stderr)
         return 3
     _ffi.load()
+    if args.reference_scenario:
+        res = reference_scale(args.shifts, args.compare, args.slices or None)
+        print(json.dumps({"workload": "reference_scenario", "n_gpus": 1, **res}), flush=True)
+        return 0
     if args.calib_copy_mb:
         src = torch.ones(args.calib_copy_mb * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)

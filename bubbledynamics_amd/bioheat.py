@@ -58,10 +58,27 @@ def pennes_params(dx: float = 2e-4, dt: float = 0.05, substeps: int = 2, k: floa
     return p
 
 
+_SPIRAL = (2e-3, 3e-3, 0.05e-3, 0.15, 1e-3, 2e-4, -0.085 + 425 * 2e-4)   # speed, scale, speed_init, t_acc, dt, dxy, start
+
+
+def spiral_reference_line_ends(lines: int = 13) -> list[int]:
+    """Number of substeps done when each line of the reference's spiral ends
+    (the loop of T:345-408: line k has increase*Nt_scale + 2*Nt_acc steps,
+    increase = 1 + k//2, the first line one fewer)."""
+    speed, scale, _, t_acc, dt = _SPIRAL[:5]
+    nt_scale = 1 + int(scale / (speed * dt))                 # T:167
+    nt_acc = 1 + int(t_acc / dt)                             # T:170
+    ends, total = [], 0
+    for k in range(lines):
+        total += (1 + k // 2) * nt_scale + 2 * nt_acc - (1 if k == 0 else 0)
+        ends.append(total)
+    return ends
+
+
 def spiral_reference(lines: int = 13) -> tuple[np.ndarray, np.ndarray]:
     """Shift sequence of the reference's continuous spiral (T:141-173, T:323-409)."""
     lib = _ffi.load()
-    args = (2e-3, 3e-3, 0.05e-3, 0.15, 1e-3, 2e-4, -0.085 + 425 * 2e-4, lines)
+    args = _SPIRAL + (lines,)
     n = lib.bd_spiral_reference(*args, None, None, 0)
     sx = (C.c_int * n)(); sy = (C.c_int * n)()
     lib.bd_spiral_reference(*args, sx, sy, n)
@@ -110,6 +127,21 @@ class BioheatRun:
         _ffi.check("bd_bioheat_run", rc)
         self.cur = which.value
         self.steps += n
+
+    def set_reference_initial(self, T0_tissue_c: float = 23.6, T0_water_c: float = 22.7) -> None:
+        """The reference's initial state (T:149-150, T:233-257): T0 by region
+        (water where iz*dz < water_tissue, T:79) in both buffers, and TD seeded
+        with one step of dose at T0, 4^(T0 - 43) dt_T."""
+        iz = 0
+        while iz < self.nz and iz * self.p.dz < self.p.water_tissue:
+            iz += 1
+        for t in self.T:
+            t[:iz] = T0_water_c + 273.15 - self.p.T_off
+            t[iz:] = T0_tissue_c + 273.15 - self.p.T_off
+        self.TD[:iz] = 4.0 ** (T0_water_c - 43.0) * self.p.dt_T
+        self.TD[iz:] = 4.0 ** (T0_tissue_c - 43.0) * self.p.dt_T
+        if self.TDc is not None:
+            self.TDc.zero_()
 
     def temperature(self) -> torch.Tensor:
         return self.T[self.cur].double() + self.p.T_off

@@ -92,7 +92,8 @@ def pennes_spiral(Q: torch.Tensor, n: int, dx: float = 2e-4, dwell_s: float = 1.
 
 def reference_scenario(nxy: int = 48, nz: int = 64, settle_periods: int = 6, periods: int = 4,
                        n_harm: int = 10, steps_per_period: int = 35, dtype=torch.float64,
-                       q3d_path: str | None = None):
+                       q3d_path: str | None = None, full_scale: bool = False, progress=None,
+                       chunk: int = 1000):
     """The reference's own heat-source computation (W:556-790) on a reduced
     grid: two-layer water/muscle medium, quarter domain with mirrors, Mur
     faces, cap source with a hole.
@@ -105,9 +106,22 @@ def reference_scenario(nxy: int = 48, nz: int = 64, settle_periods: int = 6, per
        (W:663-755; running DFT bins in place of the slab re-runs and FFT);
     4. optionally write the Q3D file (W:778-790).
     dt is set so that a period is a whole number of steps (the reference's
-    0.01/f gives 100).  Returns (Q (nz, nxy, nxy) on the GPU, params, steps)."""
-    p = presets.reference_scaled(nxy, nz)
-    p.dt = 1.0 / (p.fr * steps_per_period)
+    0.01/f gives 100).  full_scale=True takes the reference's own grid and
+    constants instead (W:356-417: 426 x 426 x 2003, dt = 0.01/f, 100 steps a
+    period; nxy, nz and steps_per_period are then ignored).  progress(stage,
+    step) is called after every `chunk` settle steps and at the end of each
+    stage.  Returns (Q (nz, nxy, nxy) on the GPU, params, steps)."""
+    if full_scale:
+        p, nxy, nz = presets.reference_westervelt()
+        steps_per_period = round(1.0 / (p.fr * p.dt))
+    else:
+        p = presets.reference_scaled(nxy, nz)
+        p.dt = 1.0 / (p.fr * steps_per_period)
+
+    def note(stage, step):
+        if progress is not None:
+            torch.cuda.synchronize()
+            progress(stage, step)
     n_settle = settle_periods * steps_per_period
     n_time = periods * steps_per_period
     total = n_settle + max(steps_per_period, n_time)
@@ -115,7 +129,9 @@ def reference_scenario(nxy: int = 48, nz: int = 64, settle_periods: int = 6, per
     code = BD_F32 if dtype == torch.float32 else BD_F64
     lib = _ffi.load()
     st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-    run.run(n_settle)
+    for s0 in range(0, n_settle, chunk):
+        run.run(min(chunk, n_settle - s0))
+        note("settle", min(s0 + chunk, n_settle))
     snapshot = run.state.clone()
     Q = torch.zeros((nz, nxy, nxy), dtype=dtype, device="cuda")
     for _ in range(steps_per_period):
@@ -127,8 +143,10 @@ def reference_scenario(nxy: int = 48, nz: int = 64, settle_periods: int = 6, per
     _ffi.check("acoustic_power_water", lib.bd_acoustic_power_water(
         C.byref(p), code, C.c_void_p(Q.data_ptr()), C.c_void_p(run.pressure(0).data_ptr()),
         C.c_void_p(run.pressure(2).data_ptr()), nxy, nz, 1, st))
+    note("water", steps_per_period)
     # restart from the kept state (W:674-703)
     run.state.copy_(snapshot)
+    del snapshot
     run.seek(n_settle)
     iz_t = int(lib.bd_westervelt_first_tissue_plane(C.byref(p), nz))
     H = torch.zeros((2 * n_harm, nz - iz_t, nxy, nxy), dtype=dtype, device="cuda")
@@ -137,6 +155,7 @@ def reference_scenario(nxy: int = 48, nz: int = 64, settle_periods: int = 6, per
         _ffi.check("harmonics", lib.bd_harmonics_accumulate(code, C.c_void_p(H.data_ptr()),
                                                             C.c_void_p(run.pressure(0).data_ptr()), nxy, nz, iz_t,
                                                             i, n_time, periods, n_harm, 0, st))
+    note("harmonics", n_time)
     _ffi.check("harmonics_q", lib.bd_harmonics_q(C.byref(p), code, C.c_void_p(Q.data_ptr()),
                                                 C.c_void_p(H.data_ptr()), nxy, nz, iz_t, n_time, n_harm, 7.2, 0,
                                                 st))

@@ -222,7 +222,11 @@ int bd_bioheat_step(const bd_bioheat_params* prm, int dtype, void* T_out, const 
 
 /* n_steps substeps of bd_bioheat_step at one source shift, alternating
  * between the fields T_a and T_b: *which (0 or 1) names the field holding the
- * current temperature on entry and on return.  Asynchronous on the stream. */
+ * current temperature on entry and on return.  Asynchronous on the stream.
+ * FP32 runs pairs of substeps in one launch (bit-identical results; each pair
+ * flips *which once, so the other field then holds the level two substeps
+ * back, and both fields must hold the same boundary values, as the
+ * reference's initialisation gives, T:233-257). */
 int bd_bioheat_run(const bd_bioheat_params* prm, int dtype, void* T_a, void* T_b, const void* Q,
                    void* TD, void* TD_comp, int Nxy, int Nz, int shift_x, int shift_y, int n_steps,
                    int* which, void* stream);

@@ -85,6 +85,46 @@ def test_pennes_fp32_offset_kahan_vs_oracle(orc):
     assert rel.max() <= 1e-3
 
 
+def _random_state(p, n, nz, dtype, kahan, offset, seed):
+    rng = np.random.default_rng(seed)
+    Q = dev(rng.uniform(0.0, 5e8, (nz, n, n)), dtype)
+    run = bh.BioheatRun(p, n, nz, Q, p.T_art, dtype=dtype, kahan=kahan, offset=offset)
+    T = dev(p.T_art - run.p.T_off + rng.uniform(-5.0, 30.0, (nz, n, n)), dtype)
+    run.T[0].copy_(T)
+    run.T[1].copy_(T)                      # both buffers hold the same boundary values
+    run.TD.copy_(dev(rng.uniform(0.0, 2.0, (nz, n, n)), dtype))
+    if kahan:
+        run.TDc.copy_(dev(rng.uniform(-1e-7, 1e-7, (nz, n, n)), dtype))
+    return run
+
+
+@pytest.mark.parametrize("dtype,kahan,offset", [(torch.float32, True, True), (torch.float32, False, False),
+                                                (torch.float64, False, False), (torch.float64, True, True)])
+@pytest.mark.parametrize("n,nz", [(37, 29), (70, 41), (131, 20)])
+def test_fused_substep_pairs_bitwise_equal(monkeypatch, dtype, kahan, offset, n, nz):
+    # bd_bioheat_run launches two substeps at a time; the temperatures, dose
+    # and compensation must be the bits of the same substeps run one at a time
+    # (two-layer medium, perfusion, shifted source, odd substep counts, grids
+    # that end inside a 62 x 14 / 62 x 6 output tile and inside a z-chunk)
+    p = two_layer(n, nz)
+    p.T_art = 310.15
+    p.perf = 2e-3
+    a = _random_state(p, n, nz, dtype, kahan, offset, 11)
+    b = _random_state(p, n, nz, dtype, kahan, offset, 11)
+    for sx, sy, m in ((3, -2, 5), (0, 0, 4), (-7, 5, 7), (40, 1, 2)):
+        monkeypatch.setenv("BD_BH_FUSE", "1")      # FP64 runs single substeps by default
+        a.step(sx, sy, m)
+        monkeypatch.setenv("BD_BH_FUSE", "0")
+        b.step(sx, sy, m)
+    monkeypatch.delenv("BD_BH_FUSE", raising=False)
+    torch.cuda.synchronize()
+    assert not torch.equal(a.T[a.cur], b.T[1 - b.cur])     # the fields moved
+    assert torch.equal(a.T[a.cur], b.T[b.cur])
+    assert torch.equal(a.TD, b.TD)
+    if kahan:
+        assert torch.equal(a.TDc, b.TDc)
+
+
 def test_offset_storage_is_exact_change_of_variables(orc):
     # FP64: storing T - T_art gives the same temperatures as kelvin storage
     n = 24
