@@ -121,6 +121,26 @@ def test_reference_spiral_library_matches_oracle(orc):
     assert not np.any((np.diff(sx) != 0) & (np.diff(sy) != 0))
 
 
+def test_reference_spiral_line_ends():
+    # the line boundaries of the reference's loop (T:345-408) split the shift
+    # sequence into legs that move along x (even lines) or y (odd lines) only,
+    # each leg taking the focus farther out than the one before in its axis
+    sx, sy = bh.spiral_reference()
+    ends = bh.spiral_reference_line_ends()
+    assert len(ends) == 13 and ends[-1] == len(sx)
+    start = 0
+    for k, e in enumerate(ends):
+        moving, fixed = (sx, sy) if k % 2 == 0 else (sy, sx)
+        lo = max(start - 1, 0)
+        assert np.all(fixed[lo:e] == fixed[lo])
+        assert np.any(np.diff(moving[lo:e]) != 0)
+        start = e
+    # the lines grow by one scale length every second line (1501 steps)
+    lens = np.diff([0] + ends)
+    assert lens[0] == 1501 + 2 * 151 - 1
+    assert all(lens[k] == (1 + k // 2) * 1501 + 2 * 151 for k in range(1, 13))
+
+
 def test_dwell_spiral():
     sx, sy = bh.spiral_dwell(7, 8e-3, 2e-4)
     pts = list(zip(sx.tolist(), sy.tolist()))
