@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -56,8 +57,12 @@ __device__ __forceinline__ T bh_dose(const BhArgs<T>& a, T Tn) {
         const float x = Tn + a.dose_off;               // T_c - 43
         return exp2f(x < 0.0f ? 2.0f * x : x) * (float)a.dt_T;
     } else {
+        // T:99-102 writes pow(4, T_c - 43) and pow(2, T_c - 43); they are
+        // evaluated as exp2(2 (T_c - 43)) and exp2(T_c - 43), the same values
+        // (2x is exact) up to the last bit of the library function, at a
+        // fraction of pow's cost (FP64 pair kernel: 66 -> 149 G cell-substeps/s)
         const double T_c = (Tn + a.T_off) - 273.15;   // T:88, T:96
-        return (T_c < 43.0) ? pow(4.0, T_c - 43.0) * a.dt_T : pow(2.0, T_c - 43.0) * a.dt_T;
+        return (T_c < 43.0) ? exp2(2.0 * (T_c - 43.0)) * a.dt_T : exp2(T_c - 43.0) * a.dt_T;
     }
 }
 
@@ -148,8 +153,8 @@ __global__ void __launch_bounds__(kBX * kBY) bh_march(BhArgs<T> a) {
 }
 
 // Two substeps in one launch (temporal blocking).  The workgroup's 64 x
-// 4*RY region computes substep 1 on every region cell and substep 2 one
-// plane behind on the inner (64-2) x (4*RY-2) cells, whose neighbours are
+// TY*RY region computes substep 1 on every region cell and substep 2 one
+// plane behind on the inner (64-2) x (TY*RY-2) cells, whose neighbours are
 // all region cells; neighbouring regions overlap by two columns / rows.
 // z: the workgroup owns output planes [z0, z1) and runs substep 1 on planes
 // z0-1 .. z1.  Substep-1 values live in registers (own column, 3 planes) and
@@ -159,15 +164,17 @@ __global__ void __launch_bounds__(kBX * kBY) bh_march(BhArgs<T> a) {
 // bit-identical to them.  Substep-1 values of grid-boundary cells (never
 // updated) are read from `dst`, which the first of two single substeps
 // would have left there; the result goes to `dst`.
-template <typename T, int RY>
-__global__ void __launch_bounds__(kBX * kBY) bh_march2(BhArgs<T> a) {
-    constexpr int ROWS = kBY * RY;
+template <typename T, int TY, int RY>
+__global__ void __launch_bounds__(kBX * TY) bh_march2(BhArgs<T> a) {
+    constexpr int ROWS = TY * RY;
+    // the y-halo rows below and above the region come from the first and last thread row
+    static_assert(TY >= 2 && RY >= 1, "two thread rows at least");
     __shared__ T sa[2][ROWS + 2][kBX + 2];    // T^n plane zs with a one-cell halo
     __shared__ T sb[2][ROWS][kBX];            // substep-1 plane
     const int lb = xcd_remap((int)blockIdx.x, a.nblocks);
     const int zc = lb % a.nzc, t2 = lb / a.nzc;
     const int bx = t2 % a.ntx, by = t2 / a.ntx;
-    const int tx = (int)threadIdx.x % kBX, ty = (int)threadIdx.x / kBX;
+    const int tx = (int)threadIdx.x % kBX, ty = (int)threadIdx.x / kBX;   // ty < TY
     const int N = a.Nxy;
     const size_t plane = (size_t)N * N;
     const uint32_t ES = sizeof(T);
@@ -202,7 +209,7 @@ __global__ void __launch_bounds__(kBX * kBY) bh_march2(BhArgs<T> a) {
         ohx[r] = (hxd != 0 && gy[r] < N && hx >= 0 && hx < N) ? ((uint32_t)gy[r] * (uint32_t)N + (uint32_t)hx) * ES
                                                               : 0xFFFFFFFFu;
     }
-    const int hyd = ty == 0 ? -1 : (ty == kBY - 1 ? 1 : 0);
+    const int hyd = ty == 0 ? -1 : (ty == TY - 1 ? 1 : 0);
     const int hyrow = hyd < 0 ? gy0 - 1 : gy0 + ROWS;
     const uint32_t ohy = (hyd != 0 && gx < N && hyrow >= 0 && hyrow < N)
                              ? ((uint32_t)hyrow * (uint32_t)N + (uint32_t)gx) * ES : 0xFFFFFFFFu;
@@ -344,12 +351,14 @@ static int launch_bh(const bd_bioheat_params* p, const T* src, T* dst, const T* 
     return BD_OK;
 }
 
-// Two fused substeps (bh_march2): FP32 regions of 64 x 16, FP64 64 x 8.
+// Two fused substeps (bh_march2): FP32 regions of 64 x 16 (4 thread rows of
+// 4 rows), FP64 64 x 8.
 template <typename T>
 static int launch_bh2(const bd_bioheat_params* p, const T* src, T* dst, const T* Q, T* TD, T* TDc, int Nxy,
                       int Nz, int sx, int sy, hipStream_t st) {
+    constexpr int TY = 4;
     constexpr int RY = sizeof(T) == 4 ? 4 : 2;
-    constexpr int ROWS = kBY * RY;
+    constexpr int ROWS = TY * RY;
     BhArgs<T> a;
     a.src = src; a.dst = dst; a.Q = Q; a.TD = TD; a.TDc = TDc;
     a.Nxy = Nxy; a.Nz = Nz;
@@ -362,15 +371,24 @@ static int launch_bh2(const bd_bioheat_params* p, const T* src, T* dst, const T*
     // z-chunk length: each workgroup runs zchunk + 2 substep-1 planes, and the
     // launch runs in ceil(workgroups / resident) rounds; take the length of
     // least rounds * (zchunk + 2), at least 8 planes (BD_BH_ZCHUNK overrides)
-    static int resident[2] = {0, 0};
-    int& res = resident[sizeof(T) == 8];
+    constexpr int kMaxDev = 64;
+    static int resident[kMaxDev][2] = {};
+    static std::mutex mtx;
+    int dev = 0, res = 0;
+    BD_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDev) return BD_ERR_INVALID;
+    {
+        std::lock_guard<std::mutex> lock(mtx);
+        res = resident[dev][sizeof(T) == 8];
+    }
     if (res == 0) {
-        int dev = 0, per_cu = 0;
+        int per_cu = 0;
         hipDeviceProp_t prop;
-        BD_HIP_CHECK(hipGetDevice(&dev));
         BD_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
-        BD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bh_march2<T, RY>, kBX * kBY, 0));
+        BD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bh_march2<T, TY, RY>, kBX * TY, 0));
         res = std::max(1, per_cu) * prop.multiProcessorCount;
+        std::lock_guard<std::mutex> lock(mtx);
+        resident[dev][sizeof(T) == 8] = res;
     }
     const long long tiles = (long long)a.ntx * a.nty;
     int zchunk = nzin;
@@ -397,7 +415,7 @@ static int launch_bh2(const bd_bioheat_params* p, const T* src, T* dst, const T*
     a.dt_T = p->dt_T;
     a.dose_off = (T)((p->T_off - 273.15) - 43.0);
     a.reference_order = 0;
-    hipLaunchKernelGGL((bh_march2<T, RY>), dim3(a.nblocks), dim3(kBX * kBY), 0, st, a);
+    hipLaunchKernelGGL((bh_march2<T, TY, RY>), dim3(a.nblocks), dim3(kBX * TY), 0, st, a);
     BD_LAUNCH_CHECK();
     return BD_OK;
 }
@@ -442,11 +460,10 @@ extern "C" int bd_bioheat_run(const bd_bioheat_params* prm, int dtype, void* T_a
         return BD_ERR_INVALID;
     hipStream_t st = as_stream(stream);
     void* f[2] = {T_a, T_b};
-    // FP32: pairs of substeps in one launch (BD_BH_FUSE=0 turns it off, =1
-    // forces it for FP64).  256^3, 1,960 substeps (G cell-substeps/s):
-    // FP32 + Kahan 153 single / 222 paired; FP64 80 single / 66-74 paired
-    // (151 VGPRs, 3 waves per SIMD, 37% of substep 1 recomputed in the halo).
-    bool fuse = dtype == BD_F32;
+    // pairs of substeps in one launch (BD_BH_FUSE=0 runs them one at a time).
+    // 256^3, 1,960 substeps (G cell-substeps/s, single / paired):
+    // FP32 + Kahan 153 / 222, FP64 87 / 149.
+    bool fuse = true;
     if (const char* v = std::getenv("BD_BH_FUSE")) fuse = std::atoi(v) != 0;
     int k = 0;
     for (; fuse && k + 2 <= n_steps; k += 2) {

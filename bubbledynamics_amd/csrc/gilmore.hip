@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -546,10 +547,14 @@ constexpr GmShape kGmDefaultShape = {256, 1, 8};
 constexpr int kGmMaxDevices = 64;
 static GmShape g_gm_tuned[kGmMaxDevices][2][2];
 static bool g_gm_has_tuned[kGmMaxDevices][2][2];
+// guards the per-device host-side state of this file (tuned shapes, scratch
+// and counter allocations) against concurrent host threads
+static std::mutex g_gm_mutex;
 
 static GmShape gm_shape(const bd_gilmore_batch* b) {
     GmShape s = kGmDefaultShape;
     int dev = 0;
+    std::lock_guard<std::mutex> lock(g_gm_mutex);
     if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < kGmMaxDevices &&
         g_gm_has_tuned[dev][b->method][b->precision])
         s = g_gm_tuned[dev][b->method][b->precision];
@@ -562,6 +567,7 @@ static GmShape gm_shape(const bd_gilmore_batch* b) {
 // one counter per device for the drop-in sweep (its signature has no workspace)
 static int* sweep_counter() {
     static int* ctr[64] = {nullptr};
+    std::lock_guard<std::mutex> lock(g_gm_mutex);
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
     if (!ctr[dev] && hipMalloc(&ctr[dev], sizeof(int)) != hipSuccess) ctr[dev] = nullptr;
@@ -673,6 +679,7 @@ struct GmSched { GmLane* state; long long* cost; int* order; int* hist; };
 static bool gm_sched_scratch(int n, GmSched* out) {
     static void* buf[kGmMaxDevices] = {nullptr};
     static size_t cap[kGmMaxDevices] = {0};
+    std::lock_guard<std::mutex> lock(g_gm_mutex);
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kGmMaxDevices) return false;
     const size_t s_state = sizeof(GmLane) * (size_t)n, s_cost = sizeof(long long) * (size_t)n;
@@ -863,8 +870,11 @@ extern "C" int bd_gilmore_autotune(const bd_gilmore_params* prm, const bd_gilmor
         BD_HIP_CHECK(hipEventElapsedTime(&t, sc.e0, sc.e1));
         if (t < best_ms) { best_ms = t; best = c; }
     }
-    g_gm_tuned[dev][b.method][b.precision] = best;
-    g_gm_has_tuned[dev][b.method][b.precision] = true;
+    {
+        std::lock_guard<std::mutex> lock(g_gm_mutex);
+        g_gm_tuned[dev][b.method][b.precision] = best;
+        g_gm_has_tuned[dev][b.method][b.precision] = true;
+    }
     if (best_block) *best_block = best.block;
     if (best_chunk) *best_chunk = best.chunk;
     if (best_waves) *best_waves = best.waves;

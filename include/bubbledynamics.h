@@ -223,7 +223,7 @@ int bd_bioheat_step(const bd_bioheat_params* prm, int dtype, void* T_out, const 
 /* n_steps substeps of bd_bioheat_step at one source shift, alternating
  * between the fields T_a and T_b: *which (0 or 1) names the field holding the
  * current temperature on entry and on return.  Asynchronous on the stream.
- * FP32 runs pairs of substeps in one launch (bit-identical results; each pair
+ * Pairs of substeps run in one launch (bit-identical results; each pair
  * flips *which once, so the other field then holds the level two substeps
  * back, and both fields must hold the same boundary values, as the
  * reference's initialisation gives, T:233-257). */
@@ -306,7 +306,9 @@ void bd_gilmore_reference_params(bd_gilmore_params* p);
  * per (frequency combo, R0) pair, idx = i_combo*N_R0 + i_R0; the inertial-
  * cavitation threshold amplitude found in 0.1 / 0.01 / 0.001 MPa stages, -1
  * if none is set, 1e7 if no collapse up to 10 MPa.  Fr_combo: 3*N_fr_combo
- * frequencies (device).  N_A is accepted and ignored, as in the reference. */
+ * frequencies (device).  N_A is accepted and ignored, as in the reference.
+ * The queue counter is one per device: sweeps that overlap in time on one
+ * device (other streams or host threads) race; run them one after another. */
 int bd_gilmore_threshold_sweep(const bd_gilmore_params* prm, double* results_A,
                                const double* Fr_combo, int N_fr_combo, int N_R0, int N_A,
                                void* stream);
@@ -352,23 +354,24 @@ typedef struct bd_gilmore_batch {
  * attempts (quarter-octave buckets, most expensive first), so the longest
  * bubbles do not start late and hold up the end of the run.  SLICED keeps a
  * per-device scratch (about 160 bytes per bubble) until the process ends and
- * synchronises the stream between passes. */
+ * synchronises the stream between passes; SLICED calls that overlap in time
+ * on one device share that scratch and race, so run them one after another. */
 #define BD_GM_SCHED_INPUT  0
 #define BD_GM_SCHED_SLICED 1
 
 /* Bytes of device workspace bd_gilmore_integrate needs (queue counter). */
+size_t bd_gilmore_workspace_bytes(void);
+/* Integrates a batch with a persistent work-queue kernel: each lane takes the
+ * next bubble from a device counter as soon as its previous one finished. */
+int bd_gilmore_integrate(const bd_gilmore_params* prm, const bd_gilmore_batch* batch,
+                         void* workspace, void* stream);
+
 /* Writes a threshold table (host arrays: results[combo * n_r0 + i_r0] as
  * bd_gilmore_threshold_sweep returns it) in the reference's output format
  * (G:580-606): header "fr1;fr2;fr3 --> R0;R0...", one ';'-separated line
  * per triplet, closing line "time (s);seconds"; numbers as printf %g. */
 int bd_gilmore_sweep_write(const char* path, const double* fr_combo, int n_combo, const double* results,
                            int n_r0, double min_r0, double step_r0, double seconds);
-
-size_t bd_gilmore_workspace_bytes(void);
-/* Integrates a batch with a persistent work-queue kernel: each lane takes the
- * next bubble from a device counter as soon as its previous one finished. */
-int bd_gilmore_integrate(const bd_gilmore_params* prm, const bd_gilmore_batch* batch,
-                         void* workspace, void* stream);
 /* Autotuner for the launch shape of bd_gilmore_integrate (no counterpart in
  * the reference): times block in {64,128,256} x waves per SIMD in {2,4,8}
  * (and bubbles per grab in {1,4} at 8 waves) on the first `sample` bubbles of
