@@ -179,7 +179,8 @@ def coupled_figures() -> dict:
             "max_cem43_min": float(run.dose().max().item() / 60.0)}
 
 
-def reference_scale(shifts: int, compare: int = -1, out_dir: str | None = None) -> dict:
+def reference_scale(shifts: int, compare: int = -1, out_dir: str | None = None, thermal_f64: bool = False,
+                    npy_dir: str | None = None) -> dict:
     """The reference's heat-source and thermal programs at their own grids."""
     import ctypes as C
     import tempfile
@@ -263,7 +264,7 @@ def reference_scale(shifts: int, compare: int = -1, out_dir: str | None = None) 
     ends = bh.spiral_reference_line_ends()
     assert ends[-1] == len(sx)
     # FP64 drop-in precision on the first `compare` substeps (default: the first spiral line)
-    m_cmp = min(ends[0] if compare < 0 else compare, n)
+    m_cmp = 0 if thermal_f64 else min(ends[0] if compare < 0 else compare, n)
     ref64 = None
     if m_cmp > 0:
         run64 = bh.BioheatRun(bp, nb, nzb, Qf, 0.0, dtype=torch.float64, kahan=False, offset=False)
@@ -275,14 +276,19 @@ def reference_scale(shifts: int, compare: int = -1, out_dir: str | None = None) 
         log(f"bioheat_fp64: {out['bioheat_fp64']}")
         del run64
         torch.cuda.empty_cache()
-    # FP32 with fields stored as T - T0 and a compensated dose, all substeps
-    p32 = bp.copy()
-    p32.T_art = T0_tissue                      # the storage offset (no perfusion in this program)
-    run = bh.BioheatRun(p32, nb, nzb, Qf, 0.0, dtype=torch.float32, kahan=True, offset=True)
+    # FP32 with fields stored as T - T0 and a compensated dose (or FP64 as the
+    # reference), all substeps
+    tag = "FP64" if thermal_f64 else "FP32"
+    if thermal_f64:
+        run = bh.BioheatRun(bp, nb, nzb, Qf, 0.0, dtype=torch.float64, kahan=False, offset=False)
+    else:
+        p32 = bp.copy()
+        p32.T_art = T0_tissue                  # the storage offset (no perfusion in this program)
+        run = bh.BioheatRun(p32, nb, nzb, Qf, 0.0, dtype=torch.float32, kahan=True, offset=True)
     del Qf
     torch.cuda.empty_cache()
     run.set_reference_initial()
-    t_b = advance(run, 0, m_cmp, "FP32")
+    t_b = advance(run, 0, m_cmp, tag)
     if ref64 is not None:
         T64, D64 = ref64
         dT = (run.temperature() - T64).abs().max().item()
@@ -293,23 +299,33 @@ def reference_scale(shifts: int, compare: int = -1, out_dir: str | None = None) 
         log(f"FP32 vs FP64 after {m_cmp} substeps: max |dT| {dT:.2e} K, dose rel err {drel:.2e}")
         del T64, D64, ref64, sel
         torch.cuda.empty_cache()
-    if out_dir:
-        os.makedirs(out_dir, exist_ok=True)
+    for d_ in (out_dir, npy_dir):
+        if d_:
+            os.makedirs(d_, exist_ok=True)
     from bubbledynamics_amd._ffi import BD_SLICE_XY, BD_SLICE_XZ
+
+    def dump(name, t, off):
+        # float64 copy of a slice for analysis (the text slices keep 6 digits)
+        np.save(os.path.join(npy_dir, name), (t.double() + off).cpu().numpy())
     # the reference's T slices after spiral lines 1, 5, 9, 13 (T:411-563)
     done = m_cmp
     for k, line in enumerate((1, 5, 9, 13)):
         stop = min(ends[line - 1], n)
         if stop > done:
-            t_b += advance(run, done, stop, "FP32")
+            t_b += advance(run, done, stop, tag)
             done = stop
         if out_dir and done == ends[line - 1]:
             run.write_slice(os.path.join(out_dir, f"T_XZ_loop{k}.txt"), BD_SLICE_XZ, 425, 1200, 1701, 325, 526)
             run.write_slice(os.path.join(out_dir, f"T_XY_loop{k}_z15.txt"), BD_SLICE_XY, 1500, 325, 526, 325, 526)
+        if npy_dir and done == ends[line - 1]:
+            t = run.T[run.cur]
+            dump(f"T_XZ_loop{k}.npy", t[1200:1701, 425, 325:526], run.p.T_off)
+            dump(f"T_XY_loop{k}_z15.npy", t[1500, 325:526, 325:526], run.p.T_off)
     if n > done:
-        t_b += advance(run, done, n, "FP32")
+        t_b += advance(run, done, n, tag)
     T = run.temperature()
-    out["bioheat"] = {"grid": [nb, nb, nzb], "dtype": "f32 (T - T0, Kahan dose)", "substeps": n,
+    out["bioheat"] = {"grid": [nb, nb, nzb], "dtype": "f64" if thermal_f64 else "f32 (T - T0, Kahan dose)",
+                      "substeps": n,
                       "of_reference": len(sx), "calls": len(runs_of(0, n)), "seconds": round(t_b, 2),
                       "mcell_steps_per_s": round(bcells * n / t_b / 1e6, 1),
                       "t_max_c": float(T.max().item() - 273.15),
@@ -321,6 +337,12 @@ def reference_scale(shifts: int, compare: int = -1, out_dir: str | None = None) 
         for iz, tag in ((1500, "z15"), (1350, "z13_5"), (1250, "z12_5")):
             run.write_slice(os.path.join(out_dir, f"TD_XY_{tag}.txt"), BD_SLICE_XY, iz, 325, 526, 325, 527, "TD")
         out["bioheat"]["slices"] = out_dir
+    if npy_dir:
+        D = run.dose()
+        dump("TD_XZ.npy", D[1200:1601, 425, 325:527], 0.0)
+        for iz, tag in ((1500, "z15"), (1350, "z13_5"), (1250, "z12_5")):
+            dump(f"TD_XY_{tag}.npy", D[iz, 325:526, 325:527], 0.0)
+        del D
     log(f"bioheat: {out['bioheat']}")
     return out
 
@@ -347,6 +369,10 @@ def main() -> int:
                          "(-1: the first spiral line, 1,802)")
     ap.add_argument("--slices", default="",
                     help="directory for the reference's end-of-run slice files in --reference-scenario")
+    ap.add_argument("--slices-npy", default="",
+                    help="directory for float64 .npy copies of the same slices in --reference-scenario")
+    ap.add_argument("--thermal-dtype", choices=("f32", "f64"), default="f32",
+                    help="precision of the thermal program in --reference-scenario")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the multi-GPU z-slab decomposition is not built; "
@@ -363,7 +389,8 @@ def main() -> int:
         return 3
     _ffi.load()
     if args.reference_scenario:
-        res = reference_scale(args.shifts, args.compare, args.slices or None)
+        res = reference_scale(args.shifts, args.compare, args.slices or None, args.thermal_dtype == "f64",
+                              args.slices_npy or None)
         print(json.dumps({"workload": "reference_scenario", "n_gpus": 1, **res}), flush=True)
         return 0
     if args.calib_copy_mb:
