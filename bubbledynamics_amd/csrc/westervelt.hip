@@ -486,7 +486,7 @@ static int forward_step_t(const bd_westervelt_params* prm, int Nxy, int Nz, cons
     a.pm3 = (const T*)q.pm3; a.pm4 = (const T*)q.pm4;
     a.r1 = (T*)q.r1; a.r1m1 = (const T*)q.r1m1; a.r1m2 = (const T*)q.r1m2;
     a.r2 = (T*)q.r2; a.r2m1 = (const T*)q.r2m1; a.r2m2 = (const T*)q.r2m2;
-    a.pmax = nullptr; a.epart = epart; a.focus = nullptr; a.d_step = nullptr; a.d_step_next = nullptr; a.focus_idx = -1;
+    a.pmax = nullptr; a.pmax_mask = 0; a.epart = epart; a.focus = nullptr; a.d_step = nullptr; a.d_step_next = nullptr; a.focus_idx = -1;
     int variant, blocks;
     launch_shape(prm, Nxy, Nz, sizeof(T) == 8, &variant, &blocks);
     a.g = make_geom(prm, Nxy, Nz, variant, blocks);
@@ -531,7 +531,7 @@ static int space6_t(const bd_westervelt_params* prm, void* Pnext, const void* P,
     a.pm3 = (const T*)Pm3; a.pm4 = (const T*)Pm4;
     a.r1 = (T*)R1; a.r1m1 = (const T*)R1m1; a.r1m2 = (const T*)R1m2;
     a.r2 = (T*)R2; a.r2m1 = (const T*)R2m1; a.r2m2 = (const T*)R2m2;
-    a.pmax = nullptr; a.epart = nullptr; a.focus = nullptr; a.d_step = nullptr; a.d_step_next = nullptr; a.focus_idx = -1;
+    a.pmax = nullptr; a.pmax_mask = 0; a.epart = nullptr; a.focus = nullptr; a.d_step = nullptr; a.d_step_next = nullptr; a.focus_idx = -1;
     int variant, blocks;
     launch_shape(prm, Nxy, Nz, sizeof(T) == 8, &variant, &blocks);
     a.g = make_geom(prm, Nxy, Nz, variant, blocks);
@@ -745,7 +745,7 @@ extern "C" int bd_westervelt_plan_bind(bd_westervelt_plan* pl, void* state, void
 
 template <typename T>
 static int plan_step(bd_westervelt_plan* pl, int n, const int* d_step, int step_arg, hipStream_t st,
-                     int* d_step_next = nullptr) {
+                     int pmax_mask, int* d_step_next = nullptr) {
     const size_t N = (size_t)pl->Nxy * pl->Nxy * pl->Nz;
     T* S = (T*)pl->state;
     auto P = [&](int m) { return S + (size_t)(((m % 6) + 6) % 6) * N; };
@@ -756,6 +756,7 @@ static int plan_step(bd_westervelt_plan* pl, int n, const int* d_step, int step_
     a.r1 = R1(n); a.r1m1 = R1(n - 1); a.r1m2 = R1(n - 2);
     a.r2 = R2(n); a.r2m1 = R2(n - 1); a.r2m2 = R2(n - 2);
     a.pmax = (T*)pl->pmax;
+    a.pmax_mask = pmax_mask;
     a.epart = pl->energy ? pl->d_epart : nullptr;
     a.focus = pl->focus;
     a.d_step = d_step;
@@ -769,13 +770,37 @@ static int plan_step(bd_westervelt_plan* pl, int n, const int* d_step, int step_
     return launch_march<T>(a, pl->variant, step_arg, st);
 }
 
+// Interior pmax levels taken by the kernel of step n in a run that started at
+// step s0: step n holds p^{n-4} .. p^{n+1}; the kernel whose new level is a
+// multiple of 6 takes the levels of its window produced in this run.
+static int pmax_window(int n, int s0) {
+    if ((n + 1) % 6 != 0) return 0;
+    return (1 << std::min(6, n + 1 - s0)) - 1;
+}
+
+// pmax = max(pmax, p^l) for the levels l0..l1 (oldest first) left after the
+// last window of a run.
+template <typename T>
+__global__ void pmax_fold_kernel(T* __restrict__ pmax, const T* __restrict__ S, size_t N, int l0, int l1) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+        const T old = pmax[i];
+        T m = old;
+        for (int l = l0; l <= l1; ++l) {
+            const T v = S[(size_t)(l % 6) * N + i];
+            if (v > m) m = v;
+        }
+        if (m > old) pmax[i] = m;
+    }
+}
+
 template <typename T>
 static int plan_run_t(bd_westervelt_plan* pl, int n_steps, int use_graph, hipStream_t st) {
     int n = pl->steps;
+    const int s0 = n;
     const int end = n + n_steps;
     // eager steps until the ring phase is 0 mod 6
     while (n < end && (!use_graph || n % 6 != 0)) {
-        int rc = plan_step<T>(pl, n, nullptr, n, st);
+        int rc = plan_step<T>(pl, n, nullptr, n, st, pmax_window(n, s0));
         if (rc) return rc;
         ++n;
         if (!use_graph) continue;
@@ -790,8 +815,11 @@ static int plan_run_t(bd_westervelt_plan* pl, int n_steps, int use_graph, hipStr
             BD_HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
             BD_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
             int rc = BD_OK;
+            // replays start at a step n = 0 mod 6 inside the run, so the last
+            // kernel's window (levels n+1 .. n+6) lies inside the run
             for (int j = 0; j < 6 && rc == BD_OK; ++j)
-                rc = plan_step<T>(pl, j, pl->d_step + k, j, cap, j == 5 ? pl->d_step + (1 - k) : nullptr);
+                rc = plan_step<T>(pl, j, pl->d_step + k, j, cap, j == 5 ? 63 : 0,
+                                  j == 5 ? pl->d_step + (1 - k) : nullptr);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(cap, &g);
             (void)hipStreamDestroy(cap);
@@ -808,11 +836,23 @@ static int plan_run_t(bd_westervelt_plan* pl, int n_steps, int use_graph, hipStr
         }
     }
     while (n < end) {
-        int rc = plan_step<T>(pl, n, nullptr, n, st);
+        int rc = plan_step<T>(pl, n, nullptr, n, st, pmax_window(n, s0));
         if (rc) return rc;
         ++n;
     }
     pl->steps = n;
+    // levels after the run's last window
+    if (pl->pmax) {
+        const int last = 6 * (end / 6);
+        const int l0 = (last >= s0 + 1 ? last : s0) + 1;
+        if (l0 <= end) {
+            const size_t N = (size_t)pl->Nxy * pl->Nxy * pl->Nz;
+            const int blocks = (int)std::min<size_t>((N + 255) / 256, 8192);
+            hipLaunchKernelGGL((pmax_fold_kernel<T>), dim3(blocks), dim3(256), 0, st, (T*)pl->pmax,
+                               (const T*)pl->state, N, l0, end);
+            BD_LAUNCH_CHECK();
+        }
+    }
     return BD_OK;
 }
 
@@ -840,6 +880,7 @@ static int plan_rerun_step_t(bd_westervelt_plan* pl, hipStream_t st) {
     a.r1 = R1(n); a.r1m1 = R1(n - 1); a.r1m2 = R1(n - 2);
     a.r2 = R2(n); a.r2m1 = R2(n - 1); a.r2m2 = R2(n - 2);
     a.pmax = (T*)pl->pmax;
+    a.pmax_mask = (n + 1) % 6 == 0 ? 63 : 0;      // the window of the step inside a replay
     a.epart = pl->energy ? pl->d_epart : nullptr;
     a.focus = pl->focus;
     a.d_step = nullptr;

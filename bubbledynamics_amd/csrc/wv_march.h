@@ -95,6 +95,10 @@ struct WvMarchArgs {
     T* __restrict__ r1; const T* __restrict__ r1m1; const T* __restrict__ r1m2;
     T* __restrict__ r2; const T* __restrict__ r2m1; const T* __restrict__ r2m2;
     T* __restrict__ pmax;  // nullable
+    // Interior pmax this step: 0 = not touched; else bit k takes level n+1-k
+    // (k = 0..5: p^{n+1}, p^n, p^{n-1} .. p^{n-4}, all in registers here), so
+    // one update every 6 steps covers every level.  Face cells update every step.
+    int pmax_mask;
     double* epart;         // [step][nblocks][2], nullable
     double* focus;         // [step], nullable
     const int* d_step;     // step base for graph replay, nullable
@@ -150,6 +154,7 @@ wv_march(WvMarchArgs<T> a, int step_arg) {
     __shared__ double red[NT / kWave];
 
     const int step = a.d_step ? (*a.d_step + step_arg) : step_arg;
+    const bool pm_upd = a.pmax != nullptr && a.pmax_mask != 0;
     if (a.d_step_next && blockIdx.x == 0 && threadIdx.x == 0) *a.d_step_next = step + 1;
     const int lb = xcd_remap((int)blockIdx.x, a.g.nblocks);
     const int zc = lb % a.g.nzc, t2 = lb / a.g.nzc;
@@ -231,7 +236,7 @@ wv_march(WvMarchArgs<T> a, int step_arg) {
 #pragma unroll
             for (int r = 0; r < RY; ++r) v.r1m1[r] = v.r1m2[r] = v.r2m1[r] = v.r2m2[r] = T(0);
         }
-        if (a.pmax) {
+        if (pm_upd) {
             const rsrc_t sm = make_rsrc(a.pmax + zo, pbytes);
 #pragma unroll
             for (int r = 0; r < RY; ++r) v.pmx[r] = bload<T>(sm, cidx[r] * ES);
@@ -374,7 +379,19 @@ wv_march(WvMarchArgs<T> a, int step_arg) {
                     bstore<T>(r1v, make_rsrc(a.r1 + zo, pbytes), o);
                     bstore<T>(r2v, make_rsrc(a.r2 + zo, pbytes), o);
                 }
-                if (a.pmax && pn > cur.pmx[r]) bstore<T>(pn, make_rsrc(a.pmax + zo, pbytes), o);
+                if (pm_upd) {
+                    // oldest level first, as one update per step would see them
+                    const int mk = a.pmax_mask;
+                    T m = cur.pmx[r];
+                    bool up = false;
+                    if ((mk & 32) && pm4 > m) { m = pm4; up = true; }
+                    if ((mk & 16) && pm3 > m) { m = pm3; up = true; }
+                    if ((mk & 8) && pm2 > m) { m = pm2; up = true; }
+                    if ((mk & 4) && pm1 > m) { m = pm1; up = true; }
+                    if ((mk & 2) && pc > m) { m = pc; up = true; }
+                    if ((mk & 1) && pn > m) { m = pn; up = true; }
+                    if (up) bstore<T>(m, make_rsrc(a.pmax + zo, pbytes), o);
+                }
                 if (a.focus && (long long)(zo + cidx[r]) == a.focus_idx) a.focus[step] = (double)pn;
                 lastp[r] = pn;
                 if (a.g.obj_on) {

@@ -324,6 +324,45 @@ def test_default_shape_rule(monkeypatch):
         del run
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_pmax_exact_after_every_run(dtype):
+    # the interior peak pressure is updated once every 6 steps over the six
+    # levels a step holds, plus a fold at the end of each run; after every run
+    # it must equal the running maximum formed here level by level, also after
+    # a restart from a kept state with pmax cleared
+    n, nz = 44, 40
+    p = presets.reference_scaled(n, nz)
+    run = WesterveltRun(p, n, nz, dtype=dtype, max_steps=200, energy=False)
+    ref_run = WesterveltRun(p, n, nz, dtype=dtype, max_steps=200, energy=False)
+    ref = torch.zeros_like(ref_run.pmax)
+    kept = None
+    for m in (7, 5, 13, 1, 6, 11, 24, 3):
+        run.run(m)
+        for _ in range(m):
+            ref_run.run(1, use_graph=False)
+            ref = torch.maximum(ref, ref_run.pressure(0))
+        if run.steps == 26:
+            kept = run.state.clone()
+        torch.cuda.synchronize()
+        assert torch.equal(run.pressure(0), ref_run.pressure(0))
+        assert torch.equal(run.pmax, ref), run.steps
+    assert torch.count_nonzero(ref) > 0.5 * ref.numel()
+    # restart at step 26 with pmax cleared: only the levels produced after it count
+    run.state.copy_(kept)
+    run.seek(26)
+    run.pmax.zero_()
+    ref_run.state.copy_(kept)
+    ref_run.seek(26)
+    ref = torch.zeros_like(ref)
+    for m in (4, 9, 17):
+        run.run(m)
+        for _ in range(m):
+            ref_run.run(1, use_graph=False)
+            ref = torch.maximum(ref, ref_run.pressure(0))
+        torch.cuda.synchronize()
+        assert torch.equal(run.pmax, ref), run.steps
+
+
 def test_tone_burst_small_fp32_vs_fp64(orc):
     # the benchmark workload's physics at 64^3: FP32 focal pressure against the
     # FP64 oracle, relative L2 over the whole record
