@@ -155,7 +155,12 @@ int bd_westervelt_plan_create(const bd_westervelt_params* prm, int dtype, int Nx
 int bd_westervelt_plan_bind(bd_westervelt_plan* plan, void* state, void* pmax, double* energy,
                             double* focus, const double* elem_amp, const double* elem_phase);
 /* Advances n_steps steps (interior + source + ABC each step).  use_graph=1
- * replays a captured 6-step hipGraph (the ring period). */
+ * replays a captured 6-step hipGraph (the ring period).  pmax is the running
+ * maximum over every level this plan produced once the call's work on the
+ * stream is done (inside a call the interior cells are updated once every 6
+ * steps over the six levels a step holds, and the call ends with a fold of
+ * the remaining levels); levels present before a call (e.g. restored after
+ * bd_westervelt_plan_seek) are not taken into pmax. */
 int bd_westervelt_plan_run(bd_westervelt_plan* plan, int n_steps, int use_graph, void* stream);
 /* Reduces the per-block energy partials of all steps run so far into the
  * bound energy buffer. */
