@@ -398,8 +398,18 @@ struct GmSweepArgs {
     double minR0, stepR0;
     double* results_A;
     const double* Fr_combo;
-    int* counter;
+    int* counter;              // [0] cells claimed, [1] cells completed (device)
+    int cell0, cell_end;       // cells idx in [cell0, cell_end), claimed in increasing order
+    int* host_flags;           // nullable, host-mapped: [0] claimed, [1] completed, [2] stop request
 };
+
+// System-scope access to the host-mapped progress words (vector memory ops).
+__device__ __forceinline__ void host_store(int* p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ int host_load(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 // Reference drive (G:40-57) as amplitudes per tone; the dual mode uses fr1 and fr3.
 __device__ __forceinline__ void ref_drive(GmDrive& d, bool triple, const double fr[3], double A) {
@@ -416,7 +426,6 @@ __device__ __forceinline__ void ref_drive(GmDrive& d, bool triple, const double 
 
 __global__ void __launch_bounds__(256) gm_sweep_kernel(GmSweepArgs a) {
     const GmConsts& K = a.K;
-    const int N_total = a.N_fr_combo * a.N_R0;
     const double A_MAX = 1.0e7;
     bool have = false;
     int idx = 0;
@@ -439,8 +448,12 @@ __global__ void __launch_bounds__(256) gm_sweep_kernel(GmSweepArgs a) {
 
     while (true) {
         if (!have) {
-            idx = atomicAdd(a.counter, 1);
-            if (idx >= N_total) break;
+            // a stop request ends the claiming; every claimed cell is completed,
+            // so the completed cells are always a prefix of the range
+            if (a.host_flags && host_load(a.host_flags + 2) != 0) break;
+            idx = a.cell0 + atomicAdd(a.counter, 1);
+            if (idx >= a.cell_end) break;
+            if (a.host_flags && ((idx - a.cell0) & 255) == 0) host_store(a.host_flags, idx - a.cell0 + 1);
             have = true;
             const int ic = idx / a.N_R0, ir = idx % a.N_R0;
             R0 = a.minR0 + ir * a.stepR0;
@@ -510,10 +523,21 @@ __global__ void __launch_bounds__(256) gm_sweep_kernel(GmSweepArgs a) {
         }
         a.results_A[idx] = result;
         have = false;
+        const int done = atomicAdd(a.counter + 1, 1) + 1;
+        if (a.host_flags && (done & 63) == 0) host_store(a.host_flags + 1, done);
     }
 }
 
-__global__ void zero_int_kernel(int* p) { if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0; }
+// After a range launch: the exact completed count and the finished mark.
+__global__ void gm_sweep_final_kernel(const int* counter, int* host_flags) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        host_store(host_flags + 1, counter[1]);
+        host_store(host_flags + 3, 1);
+    }
+}
+
+// zeroes p[0 .. blockDim.x) (a fill kernel: safe inside captured graphs)
+__global__ void zero_int_kernel(int* p) { if (blockIdx.x == 0) p[threadIdx.x] = 0; }
 
 static GmConsts make_consts(const bd_gilmore_params* g) {
     GmConsts K;
@@ -564,14 +588,34 @@ static GmShape gm_shape(const bd_gilmore_batch* b) {
     return s;
 }
 
-// one counter per device for the drop-in sweep (its signature has no workspace)
+// two counters per device for the drop-in sweep (its signature has no workspace)
 static int* sweep_counter() {
     static int* ctr[64] = {nullptr};
     std::lock_guard<std::mutex> lock(g_gm_mutex);
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
-    if (!ctr[dev] && hipMalloc(&ctr[dev], sizeof(int)) != hipSuccess) ctr[dev] = nullptr;
+    if (!ctr[dev] && hipMalloc(&ctr[dev], 2 * sizeof(int)) != hipSuccess) ctr[dev] = nullptr;
     return ctr[dev];
+}
+
+// Host-mapped progress words of the range sweep, one set per process:
+// host pointer for the caller, device pointer for the kernel.
+static int* g_sweep_flags_host = nullptr;
+static int* g_sweep_flags_dev = nullptr;
+static bool sweep_flags(int** host, int** dev) {
+    std::lock_guard<std::mutex> lock(g_gm_mutex);
+    if (!g_sweep_flags_host) {
+        void* p = nullptr;
+        if (hipHostMalloc(&p, 4 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) return false;
+        void* d = nullptr;
+        if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess) { (void)hipHostFree(p); return false; }
+        g_sweep_flags_host = (int*)p;
+        g_sweep_flags_dev = (int*)d;
+        std::memset(p, 0, 4 * sizeof(int));
+    }
+    *host = g_sweep_flags_host;
+    *dev = g_sweep_flags_dev;
+    return true;
 }
 
 }  // namespace bd
@@ -612,6 +656,18 @@ static int check_gilmore(const bd_gilmore_params* g) {
     return BD_OK;
 }
 
+static int launch_sweep(GmSweepArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(2), 0, st, a.counter);
+    BD_LAUNCH_CHECK();
+    const long long total = (long long)a.cell_end - a.cell0;
+    const int block = 64;
+    int blocks = persistent_blocks(block);
+    blocks = (int)std::min<long long>(blocks, (total + block - 1) / block);
+    hipLaunchKernelGGL(gm_sweep_kernel, dim3(blocks), dim3(block), 0, st, a);
+    BD_LAUNCH_CHECK();
+    return BD_OK;
+}
+
 extern "C" int bd_gilmore_threshold_sweep(const bd_gilmore_params* prm, double* results_A,
                                           const double* Fr_combo, int N_fr_combo, int N_R0, int N_A,
                                           void* stream) {
@@ -633,15 +689,51 @@ extern "C" int bd_gilmore_threshold_sweep(const bd_gilmore_params* prm, double* 
     a.results_A = results_A;
     a.Fr_combo = Fr_combo;
     a.counter = ctr;
-    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, st, ctr);
-    BD_LAUNCH_CHECK();
-    const long long total = (long long)N_fr_combo * N_R0;
-    const int block = 64;
-    int blocks = persistent_blocks(block);
-    blocks = (int)std::min<long long>(blocks, (total + block - 1) / block);
-    hipLaunchKernelGGL(gm_sweep_kernel, dim3(blocks), dim3(block), 0, st, a);
+    a.cell0 = 0;
+    a.cell_end = N_fr_combo * N_R0;
+    a.host_flags = nullptr;
+    return launch_sweep(a, st);
+}
+
+extern "C" int bd_gilmore_threshold_sweep_range(const bd_gilmore_params* prm, double* results_A,
+                                                const double* Fr_combo, int N_fr_combo, int N_R0, int cell0,
+                                                int cell_end, void* stream) {
+    int rc = check_gilmore(prm);
+    if (rc) return rc;
+    if (!results_A || !Fr_combo || N_fr_combo <= 0 || N_R0 <= 0) return BD_ERR_INVALID;
+    if ((long long)N_fr_combo * N_R0 > 0x7fffffffLL) return BD_ERR_INVALID;
+    if (cell0 < 0 || cell_end > N_fr_combo * N_R0 || cell0 >= cell_end) return BD_ERR_INVALID;
+    int* ctr = sweep_counter();
+    if (!ctr) return BD_ERR_ALLOC;
+    int *hf = nullptr, *df = nullptr;
+    if (!sweep_flags(&hf, &df)) return BD_ERR_ALLOC;
+    hipStream_t st = as_stream(stream);
+    // the caller's stop request (hf[2]) is left as it is: a stop set before the
+    // call makes the launch claim nothing
+    hf[0] = 0; hf[1] = 0; hf[3] = 0;
+    GmSweepArgs a;
+    a.K = make_consts(prm);
+    a.triple = prm->triple_fr;
+    a.N_fr_combo = N_fr_combo;
+    a.N_R0 = N_R0;
+    a.minR0 = prm->minR0;
+    a.stepR0 = prm->stepR0;
+    a.results_A = results_A;
+    a.Fr_combo = Fr_combo;
+    a.counter = ctr;
+    a.cell0 = cell0;
+    a.cell_end = cell_end;
+    a.host_flags = df;
+    rc = launch_sweep(a, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(gm_sweep_final_kernel, dim3(1), dim3(1), 0, st, (const int*)ctr, df);
     BD_LAUNCH_CHECK();
     return BD_OK;
+}
+
+extern "C" int* bd_gilmore_sweep_flags(void) {
+    int *hf = nullptr, *df = nullptr;
+    return sweep_flags(&hf, &df) ? hf : nullptr;
 }
 
 extern "C" size_t bd_gilmore_workspace_bytes(void) { return 256; }

@@ -74,23 +74,49 @@ def threshold_sweep(params: GilmoreParams, fr_combo: np.ndarray, n_r0: int) -> n
     return res.cpu().numpy()
 
 
-def threshold_sweep_full(params: GilmoreParams, fr_combo: np.ndarray, n_r0: int, chunks: int = 10,
-                         log=None) -> tuple[np.ndarray, float]:
-    """The reference's whole sweep (G:498-573): every triplet x n_r0 radii.
-    Run as `chunks` launches over strided triplet subsets (each mixes low- and
-    high-frequency triplets, so every launch is throughput bound), reporting
-    progress between launches.  Returns (results[combo * n_r0 + i_r0], seconds)."""
+def threshold_sweep_range(params: GilmoreParams, fr_combo: np.ndarray, n_r0: int, cell0: int = 0,
+                          cell_end: int | None = None, deadline_s: float | None = None, poll_s: float = 20.0,
+                          log=None, results: np.ndarray | None = None) -> tuple[np.ndarray, int, float]:
+    """The reference's sweep (G:498-573) over the cells [cell0, cell_end) of
+    idx = combo * n_r0 + i_r0 in one persistent launch.  Every `poll_s`
+    seconds the cells claimed and completed, as the kernel reports them, go to
+    `log`; after `deadline_s` the kernel is asked to stop claiming, and the
+    cells it had claimed are finished.  Returns (results with the done cells
+    filled in, end of the done prefix, seconds)."""
     import time
-    fr = np.ascontiguousarray(fr_combo, dtype=np.float64).reshape(-1, 3)
-    out = np.empty(len(fr) * n_r0)
+    fr_np = np.ascontiguousarray(fr_combo, dtype=np.float64).reshape(-1)
+    n_combo = fr_np.size // 3
+    total = n_combo * n_r0
+    cell_end = total if cell_end is None else cell_end
+    out = np.full(total, np.nan) if results is None else results
+    lib = _ffi.load()
+    flags = lib.bd_gilmore_sweep_flags()
+    if not flags:
+        raise RuntimeError("bd_gilmore_sweep_flags: host-mapped allocation failed")
+    fr = torch.as_tensor(fr_np, device="cuda")
+    res = torch.full((total,), float("nan"), dtype=torch.float64, device="cuda")
+    flags[2] = 0
     t0 = time.perf_counter()
-    for k in range(chunks):
-        idx = np.arange(k, len(fr), chunks)
-        res = threshold_sweep(params, fr[idx], n_r0).reshape(len(idx), n_r0)
-        out.reshape(len(fr), n_r0)[idx] = res
-        if log:
-            log(f"sweep chunk {k + 1}/{chunks}: {len(idx)} triplets, {time.perf_counter() - t0:.1f} s")
-    return out, time.perf_counter() - t0
+    call("bd_gilmore_threshold_sweep_range", C.byref(params), _vp(res), _vp(fr), n_combo, n_r0, cell0, cell_end,
+         _stream())
+    stopped = False
+    last = 0.0
+    while flags[3] == 0:            # set by the library's last kernel of the launch
+        time.sleep(0.25)
+        el = time.perf_counter() - t0
+        if log and el - last >= poll_s and flags[3] == 0:
+            last = el
+            log(f"sweep: {flags[0]} cells claimed, {flags[1]} completed of {cell_end - cell0}, {el:.0f} s")
+        if deadline_s is not None and el > deadline_s and not stopped:
+            flags[2] = 1
+            stopped = True
+            if log:
+                log("sweep: stop requested, finishing the claimed cells")
+    torch.cuda.synchronize()
+    done = int(flags[1])
+    flags[2] = 0
+    out[cell0:cell0 + done] = res[cell0:cell0 + done].cpu().numpy()
+    return out, cell0 + done, time.perf_counter() - t0
 
 
 def write_sweep_table(path: str, fr_combo: np.ndarray, results: np.ndarray, n_r0: int, min_r0: float,

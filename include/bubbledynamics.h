@@ -351,6 +351,24 @@ typedef struct bd_gilmore_batch {
     double reserved[3];
 } bd_gilmore_batch;
 
+/* The same sweep over the cells idx in [cell0, cell_end) only, for runs too
+ * long for one sitting (no counterpart in the reference; the full 27,775 x 100
+ * table).  Cells are claimed in increasing idx and every claimed cell is
+ * completed, so the cells done always form a prefix [cell0, cell0 + done).
+ * While it runs the kernel writes its progress to the host-mapped words of
+ * bd_gilmore_sweep_flags(): [0] cells claimed, [1] cells completed (both
+ * approximate while running), and after the launch [1] = the exact count and
+ * [3] = 1.  Setting [2] to nonzero asks the kernel to stop claiming cells;
+ * the call leaves [2] as the caller set it, so clear it before a new call.
+ * One range sweep at a time per process; call again only after [3] is 1. */
+int bd_gilmore_threshold_sweep_range(const bd_gilmore_params* prm, double* results_A,
+                                     const double* Fr_combo, int N_fr_combo, int N_R0, int cell0,
+                                     int cell_end, void* stream);
+/* The host-mapped progress / stop words of bd_gilmore_threshold_sweep_range
+ * (4 ints, allocated on first use, kept until the process ends); NULL on
+ * allocation failure. */
+int* bd_gilmore_sweep_flags(void);
+
 /* Queue order of bd_gilmore_integrate (the results do not depend on it):
  * INPUT hands bubbles out in input order.  SLICED runs passes of `slice`,
  * 16*slice and then unlimited attempts per bubble: a bubble still running at

@@ -32,6 +32,44 @@ def test_threshold_sweep_matches_oracle(orc):
     assert np.sum(got == ref) >= 6
 
 
+def test_threshold_sweep_range_prefix_and_stop():
+    # the range sweep gives the drop-in's values on the whole table and on a
+    # sub-range, with the exact completed count; a stop request set before the
+    # launch makes it claim nothing (the kernel exits at its first claim)
+    import ctypes as C
+    import time
+    from bubbledynamics_amd import _ffi
+    p = gm.reference_params()
+    p.tableau = 1
+    p.minR0, p.stepR0 = 1e-6, 1e-6
+    fr = np.array([[1e6, 1e6, 1e6], [1e6, 2e6, 3e6], [2e6, 3e6, 5e6]])
+    n_r0 = 4
+    full = gm.threshold_sweep(p, fr, n_r0)
+    out, end, _ = gm.threshold_sweep_range(p, fr, n_r0, poll_s=0.5)
+    assert end == 12
+    np.testing.assert_array_equal(out, full)
+    out, end, _ = gm.threshold_sweep_range(p, fr, n_r0, cell0=5, cell_end=11, poll_s=0.5)
+    assert end == 11
+    np.testing.assert_array_equal(out[5:11], full[5:11])
+    assert np.isnan(out[:5]).all() and np.isnan(out[11:]).all()
+    lib = _ffi.load()
+    flags = lib.bd_gilmore_sweep_flags()
+    res = torch.full((12,), float("nan"), dtype=torch.float64, device="cuda")
+    frd = torch.as_tensor(fr.reshape(-1), device="cuda")
+    flags[2] = 1
+    try:
+        _ffi.call("bd_gilmore_threshold_sweep_range", C.byref(p), C.c_void_p(res.data_ptr()),
+                  C.c_void_p(frd.data_ptr()), 3, n_r0, 0, 12, C.c_void_p(torch.cuda.current_stream().cuda_stream))
+        t0 = time.perf_counter()
+        while flags[3] == 0 and time.perf_counter() - t0 < 60.0:
+            time.sleep(0.05)
+        torch.cuda.synchronize()
+        assert flags[3] == 1 and flags[1] == 0
+        assert torch.isnan(res).all()
+    finally:
+        flags[2] = 0
+
+
 def _batch(n, seed, t_end_us):
     R0, amp, ph, fr = gm.config0_inputs(n, seed)
     return R0, amp, ph, fr, t_end_us * 1e-6
