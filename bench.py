@@ -10,6 +10,10 @@ before timing starts.
 Prints one JSON line.  `--gpus N` with N > 1 exits: one GPU is measured (the
 z-slab decomposition across GPUs is not built).
 
+`--threshold-sweep` runs the reference's threshold table (27,775 frequency
+triplets x 100 radii) from `--sweep-first` on, in resumable pieces
+(`--sweep-deadline`), and `--sweep-assemble` joins the pieces into its table file.
+
 `--reference-scenario` instead runs the reference's own programs at their own
 scale and prints one JSON line of stage timings: the heat source of
 Westervelt_relax_fft_Pressure_Energy.cu (426 x 426 x 2003 quarter domain,
@@ -347,6 +351,48 @@ def reference_scale(shifts: int, compare: int = -1, out_dir: str | None = None, 
     return out
 
 
+def threshold_table(first: int, deadline: float, out_dir: str, reference_tableau: bool) -> dict:
+    """The reference's threshold table (G:498-606: 27,775 triplets x 100 radii)
+    from cell `first` on, stopping to claim cells after `deadline` seconds; the
+    done cells go to out_dir/sweep_<first>_<end>.npy, the table file is written
+    by --sweep-assemble once the pieces cover it."""
+    from bubbledynamics_amd import gilmore as gm
+    p = gm.reference_params()
+    p.tableau = 0 if reference_tableau else 1
+    fr = gm.freq_table()
+    n_r0 = 100
+    total = len(fr) * n_r0
+    out, end, secs = gm.threshold_sweep_range(p, fr, n_r0, cell0=first, deadline_s=deadline or None, poll_s=30.0,
+                                              log=log)
+    part = out[first:end]
+    os.makedirs(out_dir, exist_ok=True)
+    np.save(os.path.join(out_dir, f"sweep_{first}_{end}.npy"), part)
+    found = part[(part > 0) & (part < 1e7)]
+    return {"workload": "threshold_sweep", "tableau": "reference" if reference_tableau else "Dormand-Prince 1980",
+            "cells": total, "first": first, "done_end": end, "seconds": round(secs, 1),
+            "cells_per_s": round((end - first) / secs, 1) if secs > 0 else None,
+            "no_collapse_1e7": int(np.sum(part == 1e7)), "unset_minus1": int(np.sum(part == -1.0)),
+            "threshold_mpa_median": float(np.median(found) / 1e6) if found.size else None}
+
+
+def assemble_table(out_dir: str, path: str) -> dict:
+    """Joins the sweep pieces of out_dir into the reference's table file."""
+    import glob
+    from bubbledynamics_amd import gilmore as gm
+    fr = gm.freq_table()
+    n_r0 = 100
+    res = np.full(len(fr) * n_r0, np.nan)
+    for f in sorted(glob.glob(os.path.join(out_dir, "sweep_*_*.npy"))):
+        a, b = (int(x) for x in os.path.basename(f)[6:-4].split("_"))
+        res[a:b] = np.load(f)
+    missing = int(np.isnan(res).sum())
+    if missing == 0:
+        p = gm.reference_params()
+        gm.write_sweep_table(path, fr, res, n_r0, p.minR0, p.stepR0, 0.0)
+    return {"workload": "threshold_sweep_assemble", "cells": len(res), "missing": missing,
+            "table": path if missing == 0 else None}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -373,11 +419,25 @@ def main() -> int:
                     help="directory for float64 .npy copies of the same slices in --reference-scenario")
     ap.add_argument("--thermal-dtype", choices=("f32", "f64"), default="f32",
                     help="precision of the thermal program in --reference-scenario")
+    ap.add_argument("--threshold-sweep", action="store_true",
+                    help="run the reference's threshold table from --sweep-first on (one JSON line)")
+    ap.add_argument("--sweep-first", type=int, default=0)
+    ap.add_argument("--sweep-deadline", type=float, default=0.0,
+                    help="seconds after which the sweep stops claiming cells (0: run to the end)")
+    ap.add_argument("--sweep-out", default="sweep_parts", help="directory of the sweep pieces")
+    ap.add_argument("--sweep-reference-tableau", action="store_true",
+                    help="the reference's a32 = -9/40 instead of the published Dormand-Prince tableau")
+    ap.add_argument("--sweep-assemble", default="",
+                    help="write the table file from the pieces in --sweep-out (no GPU)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the multi-GPU z-slab decomposition is not built; "
               "run with --gpus 1", file=sys.stderr)
         return 2
+
+    if args.sweep_assemble:
+        print(json.dumps(assemble_table(args.sweep_out, args.sweep_assemble)), flush=True)
+        return 0
 
     import numpy as np
     import torch
@@ -388,6 +448,10 @@ def main() -> int:
         print("no GPU visible", file=sys.stderr)
         return 3
     _ffi.load()
+    if args.threshold_sweep:
+        res = threshold_table(args.sweep_first, args.sweep_deadline, args.sweep_out, args.sweep_reference_tableau)
+        print(json.dumps({"n_gpus": 1, **res}), flush=True)
+        return 0
     if args.reference_scenario:
         res = reference_scale(args.shifts, args.compare, args.slices or None, args.thermal_dtype == "f64",
                               args.slices_npy or None)
