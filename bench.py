@@ -367,6 +367,8 @@ def threshold_table(first: int, deadline: float, out_dir: str, reference_tableau
     part = out[first:end]
     os.makedirs(out_dir, exist_ok=True)
     np.save(os.path.join(out_dir, f"sweep_{first}_{end}.npy"), part)
+    with open(os.path.join(out_dir, f"sweep_{first}_{end}.json"), "w") as f:
+        json.dump({"first": first, "end": end, "seconds": secs}, f)
     found = part[(part > 0) & (part < 1e7)]
     return {"workload": "threshold_sweep", "tableau": "reference" if reference_tableau else "Dormand-Prince 1980",
             "cells": total, "first": first, "done_end": end, "seconds": round(secs, 1),
@@ -385,10 +387,14 @@ def assemble_table(out_dir: str, path: str) -> dict:
     for f in sorted(glob.glob(os.path.join(out_dir, "sweep_*_*.npy"))):
         a, b = (int(x) for x in os.path.basename(f)[6:-4].split("_"))
         res[a:b] = np.load(f)
+    seconds = 0.0
+    for f in glob.glob(os.path.join(out_dir, "sweep_*_*.json")):
+        with open(f) as fh:
+            seconds += json.load(fh)["seconds"]
     missing = int(np.isnan(res).sum())
     if missing == 0:
         p = gm.reference_params()
-        gm.write_sweep_table(path, fr, res, n_r0, p.minR0, p.stepR0, 0.0)
+        gm.write_sweep_table(path, fr, res, n_r0, p.minR0, p.stepR0, seconds)
     return {"workload": "threshold_sweep_assemble", "cells": len(res), "missing": missing,
             "table": path if missing == 0 else None}
 

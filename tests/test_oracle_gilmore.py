@@ -182,3 +182,21 @@ def test_threshold_is_a_threshold(orc, g):
     assert A == pytest.approx(round(A / 1e3) * 1e3, abs=1e-6)
     assert _collapses(orc, g, R0, fr[0], A)
     assert not _collapses(orc, g, R0, fr[0], A - 1e3)
+
+
+def test_full_threshold_table_samples_match_oracle(orc):
+    # 32 cells of the full 27,775 x 100 threshold table as the GPU computed it
+    # (fixture written by tests/golden/make_threshold_samples.py from a
+    # `bench.py --threshold-sweep` run): the oracle's search gives the same
+    # amplitude, bit for bit
+    import json
+    from pathlib import Path
+    from bubbledynamics_amd import gilmore as gm
+    fx = json.loads((Path(__file__).parent / "golden" / "threshold_table_samples.json").read_text())
+    p = gm.reference_params()
+    p.tableau = 1
+    fr = gm.freq_table()
+    n_r0 = fx["n_r0"]
+    for idx, want in zip(fx["cells"], fx["gpu_threshold_pa"]):
+        got = orc.gilmore_threshold(p, fr, n_r0, idx, idx + 1, nthreads=1)[idx]
+        assert got == want, (idx, fr[idx // n_r0], got, want)
