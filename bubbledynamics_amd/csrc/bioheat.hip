@@ -152,9 +152,9 @@ __global__ void __launch_bounds__(kBX * kBY) bh_march(BhArgs<T> a) {
     }
 }
 
-// Two substeps in one launch (temporal blocking).  The workgroup's 64 x
+// Two substeps in one launch (temporal blocking).  The workgroup's BX x
 // TY*RY region computes substep 1 on every region cell and substep 2 one
-// plane behind on the inner (64-2) x (TY*RY-2) cells, whose neighbours are
+// plane behind on the inner (BX-2) x (TY*RY-2) cells, whose neighbours are
 // all region cells; neighbouring regions overlap by two columns / rows.
 // z: the workgroup owns output planes [z0, z1) and runs substep 1 on planes
 // z0-1 .. z1.  Substep-1 values live in registers (own column, 3 planes) and
@@ -164,22 +164,22 @@ __global__ void __launch_bounds__(kBX * kBY) bh_march(BhArgs<T> a) {
 // bit-identical to them.  Substep-1 values of grid-boundary cells (never
 // updated) are read from `dst`, which the first of two single substeps
 // would have left there; the result goes to `dst`.
-template <typename T, int TY, int RY>
-__global__ void __launch_bounds__(kBX * TY) bh_march2(BhArgs<T> a) {
+template <typename T, int BX, int TY, int RY>
+__global__ void __launch_bounds__(BX * TY) bh_march2(BhArgs<T> a) {
     constexpr int ROWS = TY * RY;
     // the y-halo rows below and above the region come from the first and last thread row
     static_assert(TY >= 2 && RY >= 1, "two thread rows at least");
-    __shared__ T sa[2][ROWS + 2][kBX + 2];    // T^n plane zs with a one-cell halo
-    __shared__ T sb[2][ROWS][kBX];            // substep-1 plane
+    __shared__ T sa[2][ROWS + 2][BX + 2];    // T^n plane zs with a one-cell halo
+    __shared__ T sb[2][ROWS][BX];            // substep-1 plane
     const int lb = xcd_remap((int)blockIdx.x, a.nblocks);
     const int zc = lb % a.nzc, t2 = lb / a.nzc;
     const int bx = t2 % a.ntx, by = t2 / a.ntx;
-    const int tx = (int)threadIdx.x % kBX, ty = (int)threadIdx.x / kBX;   // ty < TY
+    const int tx = (int)threadIdx.x % BX, ty = (int)threadIdx.x / BX;   // ty < TY
     const int N = a.Nxy;
     const size_t plane = (size_t)N * N;
     const uint32_t ES = sizeof(T);
     const uint32_t pbytes = (uint32_t)(plane * ES);
-    const int gx = bx * (kBX - 2) + tx;
+    const int gx = bx * (BX - 2) + tx;
     const int gy0 = by * (ROWS - 2);
     const int z0 = 1 + zc * a.zchunk;
     const int z1 = min(z0 + a.zchunk, a.Nz - 1);
@@ -194,14 +194,14 @@ __global__ void __launch_bounds__(kBX * TY) bh_march2(BhArgs<T> a) {
         gy[r] = gy0 + ly;
         ing[r] = gx < N && gy[r] < N;
         itr[r] = gx >= 1 && gx <= N - 2 && gy[r] >= 1 && gy[r] <= N - 2;
-        outc[r] = itr[r] && tx >= 1 && tx <= kBX - 2 && ly >= 1 && ly <= ROWS - 2;
+        outc[r] = itr[r] && tx >= 1 && tx <= BX - 2 && ly >= 1 && ly <= ROWS - 2;
         o[r] = ing[r] ? ((uint32_t)gy[r] * (uint32_t)N + (uint32_t)gx) * ES : 0u;
         const int ixa = gx - a.shift_x, iya = gy[r] - a.shift_y;
         const bool q_in = itr[r] && ixa >= 0 && ixa < N && iya >= 0 && iya < N;   // T:76
         cq[r] = q_in ? ((uint32_t)iya * (uint32_t)N + (uint32_t)ixa) * ES : 0xFFFFFFFFu;
     }
     // halo of the region: column -1 / 64 (edge lanes, every row), row -1 / ROWS
-    const int hxd = tx == 0 ? -1 : (tx == kBX - 1 ? 1 : 0);
+    const int hxd = tx == 0 ? -1 : (tx == BX - 1 ? 1 : 0);
     uint32_t ohx[RY];
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
@@ -356,6 +356,7 @@ static int launch_bh(const bd_bioheat_params* p, const T* src, T* dst, const T* 
 template <typename T>
 static int launch_bh2(const bd_bioheat_params* p, const T* src, T* dst, const T* Q, T* TD, T* TDc, int Nxy,
                       int Nz, int sx, int sy, hipStream_t st) {
+    constexpr int BX = 64;
     constexpr int TY = 4;
     constexpr int RY = sizeof(T) == 4 ? 4 : 2;
     constexpr int ROWS = TY * RY;
@@ -365,7 +366,7 @@ static int launch_bh2(const bd_bioheat_params* p, const T* src, T* dst, const T*
     a.iz_tissue = first_tissue_plane_bh(p, Nz);
     a.shift_x = sx; a.shift_y = sy;
     const int nin = Nxy - 2;
-    a.ntx = (nin + (kBX - 2) - 1) / (kBX - 2);
+    a.ntx = (nin + (BX - 2) - 1) / (BX - 2);
     a.nty = (nin + (ROWS - 2) - 1) / (ROWS - 2);
     const int nzin = Nz - 2;
     // z-chunk length: each workgroup runs zchunk + 2 substep-1 planes, and the
@@ -385,7 +386,7 @@ static int launch_bh2(const bd_bioheat_params* p, const T* src, T* dst, const T*
         int per_cu = 0;
         hipDeviceProp_t prop;
         BD_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
-        BD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bh_march2<T, TY, RY>, kBX * TY, 0));
+        BD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bh_march2<T, BX, TY, RY>, BX * TY, 0));
         res = std::max(1, per_cu) * prop.multiProcessorCount;
         std::lock_guard<std::mutex> lock(mtx);
         resident[dev][sizeof(T) == 8] = res;
@@ -415,7 +416,7 @@ static int launch_bh2(const bd_bioheat_params* p, const T* src, T* dst, const T*
     a.dt_T = p->dt_T;
     a.dose_off = (T)((p->T_off - 273.15) - 43.0);
     a.reference_order = 0;
-    hipLaunchKernelGGL((bh_march2<T, TY, RY>), dim3(a.nblocks), dim3(kBX * TY), 0, st, a);
+    hipLaunchKernelGGL((bh_march2<T, BX, TY, RY>), dim3(a.nblocks), dim3(BX * TY), 0, st, a);
     BD_LAUNCH_CHECK();
     return BD_OK;
 }
