@@ -429,7 +429,8 @@ def main() -> int:
                     help="run the reference's threshold table from --sweep-first on (one JSON line)")
     ap.add_argument("--sweep-first", type=int, default=0)
     ap.add_argument("--sweep-deadline", type=float, default=0.0,
-                    help="seconds after which the sweep stops claiming cells (0: run to the end)")
+                    help="seconds after which the sweep stops claiming cells (0: run to the end); "
+                         "cells already claimed still run to their end")
     ap.add_argument("--sweep-out", default="sweep_parts", help="directory of the sweep pieces")
     ap.add_argument("--sweep-reference-tableau", action="store_true",
                     help="the reference's a32 = -9/40 instead of the published Dormand-Prince tableau")
