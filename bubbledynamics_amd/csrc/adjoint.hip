@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(kAX * kAY) adj_step_kernel(AdjArgs<T> a) {
     static_assert(kAY >= 3 && kAX >= 6, "tile too small for the 3-cell halo");
     __shared__ T s[2][kAY + 6][kAX + 6];
     const int lb = xcd_remap((int)blockIdx.x, a.nblocks);
-    const int zc = lb % a.nzc, t2 = lb / a.nzc;
+    const int zc = lb / (a.ntx * a.nty), t2 = lb % (a.ntx * a.nty);   // tile-major
     const int bx = t2 % a.ntx, by = t2 / a.ntx;
     const int tx = threadIdx.x % kAX, ty = threadIdx.x / kAX;
     const int N = a.N, Nz = a.Nz;

@@ -86,7 +86,7 @@ template <typename T>
 __global__ void __launch_bounds__(kBX * kBY) bh_march(BhArgs<T> a) {
     __shared__ T s[2][kBY + 2][kBX + 2];
     const int lb = xcd_remap((int)blockIdx.x, a.nblocks);
-    const int zc = lb % a.nzc, t2 = lb / a.nzc;
+    const int zc = lb / (a.ntx * a.nty), t2 = lb % (a.ntx * a.nty);   // tile-major
     const int bx = t2 % a.ntx, by = t2 / a.ntx;
     const int tx = (int)threadIdx.x % kBX, ty = (int)threadIdx.x / kBX;
     const int ix = 1 + bx * kBX + tx;
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(BX * TY) bh_march2(BhArgs<T> a) {
     __shared__ T sa[2][ROWS + 2][BX + 2];    // T^n plane zs with a one-cell halo
     __shared__ T sb[2][ROWS][BX];            // substep-1 plane
     const int lb = xcd_remap((int)blockIdx.x, a.nblocks);
-    const int zc = lb % a.nzc, t2 = lb / a.nzc;
+    const int zc = lb / (a.ntx * a.nty), t2 = lb % (a.ntx * a.nty);   // tile-major
     const int bx = t2 % a.ntx, by = t2 / a.ntx;
     const int tx = (int)threadIdx.x % BX, ty = (int)threadIdx.x / BX;   // ty < TY
     const int N = a.Nxy;

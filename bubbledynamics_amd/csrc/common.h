@@ -33,7 +33,8 @@ constexpr int kXcds = 8;    // MI355X: 8 XCDs, workgroups dispatched round-robin
 
 // Workgroups are handed to the XCDs round-robin (block b -> XCD b % 8).  This
 // remap gives every XCD a contiguous range of logical tile ids so that tiles
-// that share halo data also share an L2.  Bijective for any nblocks.
+// that share halo data also share an L2 (the stencil kernels number their
+// tiles x, then y, then z-chunk).  Bijective for any nblocks.
 __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
     const int q = nblocks / kXcds, r = nblocks % kXcds;
     const int x = bid % kXcds, k = bid / kXcds;

@@ -157,7 +157,9 @@ wv_march(WvMarchArgs<T> a, int step_arg) {
     const bool pm_upd = a.pmax != nullptr && a.pmax_mask != 0;
     if (a.d_step_next && blockIdx.x == 0 && threadIdx.x == 0) *a.d_step_next = step + 1;
     const int lb = xcd_remap((int)blockIdx.x, a.g.nblocks);
-    const int zc = lb % a.g.nzc, t2 = lb / a.g.nzc;
+    // tile-major: consecutive ids are neighbouring tiles of one z-chunk, whose
+    // x/y halos overlap, so one XCD works on them together
+    const int zc = lb / (a.g.ntx * a.g.nty), t2 = lb % (a.g.ntx * a.g.nty);
     const int bx = t2 % a.g.ntx, by = t2 / a.g.ntx;
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     const int N = a.g.Nxy;
