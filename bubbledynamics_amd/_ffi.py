@@ -70,7 +70,8 @@ class GilmoreParams(C.Structure):
         ("triple_fr", C.c_int32), ("tableau", C.c_int32),
         ("minA", C.c_double), ("stepA", C.c_double), ("minR0", C.c_double), ("stepR0", C.c_double),
         ("abs_eps", C.c_double), ("rel_eps", C.c_double), ("h0", C.c_double),
-        ("max_solve_steps", C.c_double), ("reserved", C.c_double * 7),
+        ("max_solve_steps", C.c_double), ("arithmetic", C.c_int32), ("pad0", C.c_int32),
+        ("reserved", C.c_double * 6),
     ]
 
     def copy(self) -> "GilmoreParams":
@@ -119,6 +120,9 @@ BD_GM_FP64 = 0
 BD_GM_SPLIT = 1
 BD_GM_SCHED_INPUT = 0
 BD_GM_SCHED_SLICED = 1
+BD_GM_ARITH_REFERENCE = 0
+BD_GM_ARITH_FAST = 1
+BD_GM_SWEEP_CAPPED = -2.0
 
 _VP = C.c_void_p
 _I = C.c_int
@@ -165,6 +169,11 @@ SIGNATURES: dict[str, tuple] = {
     "bd_gilmore_threshold_sweep": (_I, [C.POINTER(GilmoreParams), _VP, _VP, _I, _I, _I, _VP]),
     "bd_gilmore_threshold_sweep_range": (_I, [C.POINTER(GilmoreParams), _VP, _VP, _I, _I, _I, _I, _VP]),
     "bd_gilmore_sweep_flags": (_IP, []),
+    "bd_gilmore_sweep_create": (_I, [C.POINTER(GilmoreParams), _VP, _I, _I, _I, _I, C.POINTER(_VP)]),
+    "bd_gilmore_sweep_advance": (_I, [_VP, _VP, C.c_longlong, _VP]),
+    "bd_gilmore_sweep_progress": (_I, [_VP, C.POINTER(C.c_longlong)]),
+    "bd_gilmore_sweep_stop_word": (_IP, [_VP]),
+    "bd_gilmore_sweep_destroy": (_I, [_VP]),
     "bd_gilmore_sweep_write": (_I, [C.c_char_p, C.POINTER(C.c_double), _I, C.POINTER(C.c_double), _I,
                                     C.c_double, C.c_double, C.c_double]),
     "bd_gilmore_workspace_bytes": (C.c_size_t, []),

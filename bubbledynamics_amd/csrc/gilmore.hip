@@ -2,19 +2,26 @@
 //
 // Reference: Gilmore_liver_3fr_combo.cu (G:line).
 //
-// Right-hand side (G:40-179), with the common subexpressions shared:
-//   p, dp/dt        one sincos per tone
-//   p_in, dp_in/dt  p_in = (P0 + 2s/R0) exp(3g log(R0/R)),  dp_in/dt = -3g p_in R'/R
-//   I^((n-1)/n) - 1 = expm1(((n-1)/n) log1p((u - v)/v)), u - v = p_in - 2s/R + tau - P0 - p
-//                   (B cancels exactly, so nothing cancels when I ~ 1)
-//   C = c_inf sqrt(I^((n-1)/n)),  I^(-1/n) = I^((n-1)/n) / I
-// Integrators: DP5(4) with the step controller of G:212-318 (the last stage
-// is reused as the next step's first, which the reference recomputes) and
-// classical RK4.  Kernels are persistent: every lane advances its own bubble
-// by one Runge-Kutta attempt per loop trip and takes the next bubble from a
-// device counter as soon as it finishes, so uneven step counts do not idle
-// lanes.  State, error norm and controller are FP64; with BD_GM_SPLIT the
-// stage right-hand sides run in FP32 while R/R0 >= split_ratio.
+// Two arithmetics of the right-hand side (G:40-179):
+//  * reference: the reference's expressions in its evaluation order (the
+//    cancelling pow(I, (n-1)/n) - 1 and (p_in - 2s/R + tau + B)/(P0 + p + B)
+//    included), with every step's first stage recomputed (G:259-273).  Only
+//    the libm rounding of pow/sin/cos is left open against the reference;
+//    threshold-sweep results are compared exactly with the FP64 oracle.
+//  * fast: common subexpressions shared,
+//      p, dp/dt        one sincos per tone
+//      p_in, dp_in/dt  p_in = (P0 + 2s/R0) exp(3g log(R0/R)),  dp_in/dt = -3g p_in R'/R
+//      I^((n-1)/n) - 1 = expm1(((n-1)/n) log1p((u - v)/v)), u - v = p_in - 2s/R + tau - P0 - p
+//                      (B cancels exactly, so nothing cancels when I ~ 1)
+//      C = c_inf sqrt(I^((n-1)/n)),  I^(-1/n) = I^((n-1)/n) / I
+//    and the last stage of a DP5(4) step reused as the next first stage
+//    (FSAL): 6 right-hand sides per attempt instead of 7.
+// Integrators: DP5(4) with the step controller of G:212-318 and classical
+// RK4.  Kernels are persistent: every lane advances its own bubble (or
+// threshold-search cell) by one Runge-Kutta attempt per loop trip and takes
+// the next one from a device counter as soon as it finishes, so uneven step
+// counts do not idle lanes.  State, error norm and controller are FP64; with
+// BD_GM_SPLIT the fast stage right-hand sides run in FP32 while R/R0 >= split_ratio.
 #include "common.h"
 #include <cstdio>
 #include <algorithm>
@@ -47,6 +54,18 @@ static GmModel<U> make_model(const bd_gilmore_params* g) {
     m.gamma3 = (U)g->gamma3;
     m.mu4 = (U)g->mu4;
     m.G4_3 = (U)g->G4_3;
+    return m;
+}
+
+// The reference's constants as its expressions use them (G:13-23).
+struct GmRefModel {
+    double rho, P0, S2, C_inf, lambda, n, B, gamma3, mu4, G4_3;
+};
+
+static GmRefModel make_ref_model(const bd_gilmore_params* g) {
+    GmRefModel m;
+    m.rho = g->rho; m.P0 = g->P0; m.S2 = g->S2; m.C_inf = g->C_inf; m.lambda = g->lambda;
+    m.n = g->n; m.B = g->B; m.gamma3 = g->gamma3; m.mu4 = g->mu4; m.G4_3 = g->G4_3;
     return m;
 }
 
@@ -83,6 +102,7 @@ static GmTab make_tab(int which) {
 struct GmConsts {
     GmModel<double> m64;
     GmModel<float> m32;
+    GmRefModel ref;
     GmTab tab;
     double C_inf, abs_eps, rel_eps, h0;
     long long cap;
@@ -115,7 +135,8 @@ __device__ __forceinline__ void drive(const GmDrive& d, double t, U& p, U& pd) {
     }
 }
 
-// State derivative (R', R'', tau', q') of G:161-179 in arithmetic U.
+// State derivative (R', R'', tau', q') of G:161-179 in arithmetic U, shared
+// subexpressions (the fast arithmetic).
 template <typename U>
 __device__ __forceinline__ void gm_rhs(const GmModel<U>& m, const GmDrive& d, double t, U R0, U Pin0,
                                        const U y[4], U f[4]) {
@@ -157,7 +178,7 @@ __device__ __forceinline__ void gm_rhs(const GmModel<U>& m, const GmDrive& d, do
     f[3] = q1;
 }
 
-// RHS on the FP64 state, evaluated in U.
+// RHS on the FP64 state, evaluated in U (fast arithmetic).
 template <typename U>
 __device__ __forceinline__ void rhs_at(const GmConsts& K, const GmDrive& d, double t, double R0, double Pin0,
                                        const double y[4], double f[4]) {
@@ -172,13 +193,88 @@ __device__ __forceinline__ void rhs_at(const GmConsts& K, const GmDrive& d, doub
     for (int c = 0; c < 4; ++c) f[c] = (double)fu[c];
 }
 
-// One DP5(4) attempt from (t, y) with step h and first stage k1 (G:246-294).
-// Produces the 5th-order solution Y, the last stage k7 = f(t+h, .) and the
-// RMS error norm.  Sums follow the reference's accumulation order.
-template <typename U>
-__device__ __forceinline__ double dp_attempt(const GmConsts& K, const GmDrive& d, double R0, double Pin0,
-                                             double t, const double y[4], double h, const double k1[4],
-                                             double Y[4], double k7[4]) {
+// ---------------------------------------------------------------------------
+// reference arithmetic (G:40-179 in the reference's evaluation order)
+// ---------------------------------------------------------------------------
+// Drive: mode 0 one tone f[0] (G:42-47), 1 dual f[0] and f[2] with A/sqrt 2
+// (G:51-52), 2 triple with A/sqrt 3 (G:55), 3 the batch's general drive
+// sum_k a[k] sin(2 pi f[k] t + ph[k]).
+struct GmRefDrive {
+    int mode;
+    double A;
+    double f[3], a[3], ph[3];
+};
+
+__device__ __forceinline__ double ref_P(const GmRefDrive& d, double t) {            // G:40-57
+    if (d.mode == 0) return d.A * sin(2.0 * M_PI * d.f[0] * t);
+    if (d.mode == 1) return (d.A / sqrt(2.0)) * (sin(2.0 * M_PI * d.f[0] * t) + sin(2.0 * M_PI * d.f[2] * t));
+    if (d.mode == 2)
+        return (d.A / sqrt(3.0)) *
+               (sin(2.0 * M_PI * d.f[0] * t) + sin(2.0 * M_PI * d.f[1] * t) + sin(2.0 * M_PI * d.f[2] * t));
+    double s = 0.0;
+    for (int k = 0; k < 3; ++k) s = s + d.a[k] * sin(2.0 * M_PI * d.f[k] * t + d.ph[k]);
+    return s;
+}
+
+__device__ __forceinline__ double ref_P_dt(const GmRefDrive& d, double t) {         // G:61-78
+    if (d.mode == 0) return d.A * 2.0 * M_PI * d.f[0] * cos(2.0 * M_PI * d.f[0] * t);
+    if (d.mode == 1)
+        return d.A * sqrt(2.0) * M_PI *
+               (d.f[0] * cos(2.0 * M_PI * d.f[0] * t) + d.f[2] * cos(2.0 * M_PI * d.f[2] * t));
+    if (d.mode == 2)
+        return d.A * (2.0 / sqrt(3.0)) * M_PI *
+               (d.f[0] * cos(2.0 * M_PI * d.f[0] * t) + d.f[1] * cos(2.0 * M_PI * d.f[1] * t) +
+                d.f[2] * cos(2.0 * M_PI * d.f[2] * t));
+    double s = 0.0;
+    for (int k = 0; k < 3; ++k) s = s + d.a[k] * 2.0 * M_PI * d.f[k] * cos(2.0 * M_PI * d.f[k] * t + d.ph[k]);
+    return s;
+}
+
+// (R', R'', tau', q') with every expression as the reference writes it.  A
+// value the reference computes more than once from the same operands (p,
+// p_in, I, tau', q' inside and outside F_R) is computed once here: the same
+// operations on the same operands give the same bits.  pow(x, 2) is x*x,
+// which equals a correctly rounded pow.
+__device__ __forceinline__ void gm_rhs_ref(const GmRefModel& m, const GmRefDrive& d, double t, double R0,
+                                           const double y[4], double f[4]) {
+    const double R = y[0], R1 = y[1], Tau = y[2], q = y[3];
+    const double p = ref_P(d, t);
+    const double p_dt = ref_P_dt(d, t);
+    const double p_in = (m.P0 + (m.S2 / R0)) * pow(R0 / R, m.gamma3);                        // G:82-85
+    const double el = -m.G4_3 * (1.0 - pow(R0 / R, 3.0));
+    const double Tau1 = (el - (m.mu4 * R1 / R) - Tau) / m.lambda;                              // G:106-110
+    const double q1 = (el - (m.mu4 * R1 / R)) / (3.0 * m.lambda) - (q / m.lambda) - (R1 * Tau / R);  // G:113-117
+    const double u = p_in - (m.S2 / R) + Tau + m.B;                                           // G:100, 125, 146
+    const double v = m.P0 + p + m.B;
+    const double I = u / v;
+    const double C = m.C_inf * pow(I, (m.n - 1.0) / (2.0 * m.n));                              // G:96-103
+    const double part1 = (m.P0 + p + m.B) * m.n / (m.rho * (m.n - 1.0));
+    const double part2 = pow(I, (m.n - 1.0) / m.n) - 1.0;
+    const double H = part1 * part2;                                                            // G:121-128
+    const double part1_dt = p_dt * m.n / (m.rho * (m.n - 1.0));
+    const double d_pin = (m.P0 + (m.S2 / R0)) * pow(R0, m.gamma3) * (-m.gamma3) * pow(R, -m.gamma3 - 1.0) * R1;  // G:89-92
+    const double u_dt = d_pin + (m.S2 * R1 / (R * R)) + Tau1;
+    const double I_dt = (u_dt * v - u * p_dt) / (v * v);
+    const double part2_dt = ((m.n - 1.0) / m.n) * pow(I, -1.0 / m.n) * I_dt;
+    const double H1 = part1_dt * part2 + part1 * part2_dt;                                    // G:132-157
+    const double a = 1.0 - (R1 / C);
+    const double b = 1.0 + (R1 / C);
+    const double t1 = 1.5 * (R1 * R1) * (1.0 - (R1 / (3.0 * C)));
+    const double t2 = b * (H + ((-Tau + 3.0 * q) / m.rho));
+    const double t3 = (H1 * a + ((-Tau1 + 3.0 * q1) / m.rho)) * R / C;
+    f[0] = R1;
+    f[1] = (-t1 + t2 + t3) / (a * R);                                                          // G:161-179
+    f[2] = Tau1;
+    f[3] = q1;
+}
+
+// One DP5(4) attempt from (t, y) with step h and first stage k1 (G:246-294);
+// rhs(t, y, f) evaluates the right-hand side.  Produces the 5th-order
+// solution Y, the last stage k7 = f(t+h, .) and the RMS error norm.  Sums
+// follow the reference's accumulation order.
+template <typename F>
+__device__ __forceinline__ double dp_attempt(const GmConsts& K, const F& rhs, double t, const double y[4], double h,
+                                             const double k1[4], double Y[4], double k7[4]) {
     double k[7][4];
     double E[4];
 #pragma unroll
@@ -197,7 +293,7 @@ __device__ __forceinline__ double dp_attempt(const GmConsts& K, const GmDrive& d
             for (int j = 0; j < i; ++j) acc = acc + K.tab.a[i][j] * k[j][c] * h;
             ys[c] = y[c] + acc;
         }
-        rhs_at<U>(K, d, t + h * K.tab.c[i], R0, Pin0, ys, k[i]);
+        rhs(t + h * K.tab.c[i], ys, k[i]);
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             Y[c] = Y[c] + K.tab.b[i] * k[i][c] * h;
@@ -217,20 +313,19 @@ __device__ __forceinline__ double dp_attempt(const GmConsts& K, const GmDrive& d
 }
 
 // Classical RK4 step (same operation order as the oracle).
-template <typename U>
-__device__ __forceinline__ void rk4_step(const GmConsts& K, const GmDrive& d, double R0, double Pin0, double t,
-                                         double h, double y[4]) {
+template <typename F>
+__device__ __forceinline__ void rk4_step(const F& rhs, double t, double h, double y[4]) {
     double k1[4], k2[4], k3[4], k4[4], s[4];
-    rhs_at<U>(K, d, t, R0, Pin0, y, k1);
+    rhs(t, y, k1);
 #pragma unroll
     for (int i = 0; i < 4; ++i) s[i] = y[i] + 0.5 * h * k1[i];
-    rhs_at<U>(K, d, t + 0.5 * h, R0, Pin0, s, k2);
+    rhs(t + 0.5 * h, s, k2);
 #pragma unroll
     for (int i = 0; i < 4; ++i) s[i] = y[i] + 0.5 * h * k2[i];
-    rhs_at<U>(K, d, t + 0.5 * h, R0, Pin0, s, k3);
+    rhs(t + 0.5 * h, s, k3);
 #pragma unroll
     for (int i = 0; i < 4; ++i) s[i] = y[i] + h * k3[i];
-    rhs_at<U>(K, d, t + h, R0, Pin0, s, k4);
+    rhs(t + h, s, k4);
 #pragma unroll
     for (int i = 0; i < 4; ++i) y[i] = y[i] + h / 6.0 * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
 }
@@ -252,7 +347,7 @@ struct GmLane {
 struct GmBatchArgs {
     GmConsts K;
     int method, precision, n, out_every, n_out, chunk;
-    double w[3];
+    double w[3], f[3];
     double t_end, dt, split_ratio;
     long long max_steps;
     const double* R0; const double* amp; const double* phase;
@@ -270,19 +365,26 @@ struct GmBatchArgs {
     int resume;
 };
 
-template <int METHOD, int PREC>
+template <int METHOD, int PREC, int ARITH>
 __global__ void __launch_bounds__(256) gm_batch_kernel(GmBatchArgs a) {
+    static_assert(ARITH == BD_GM_ARITH_FAST || PREC == BD_GM_FP64, "reference arithmetic is FP64 only");
     const GmConsts& K = a.K;
     int next = 0, end = 0;          // this lane's current chunk of bubble indices
     bool have = false;
     int i = 0;
     GmDrive d;
+    GmRefDrive rd;
     double R0 = 0, Pin0 = 0, t = 0, h = 0, rmax = 0, tc = -1, fac_max = 5.0;
     double y[4] = {0, 0, 0, 0}, k1[4] = {0, 0, 0, 0};
     bool k1_ok = false, at_start = true;
     long long acc = 0, rej = 0, s_rk = 0, nsteps_rk = 0;
     long long sl = 0;               // attempts of the current bubble in this launch
     int j_out = 0;
+    auto f64 = [&](double tt, const double* s, double* o) {
+        if constexpr (ARITH == BD_GM_ARITH_REFERENCE) gm_rhs_ref(K.ref, rd, tt, R0, s, o);
+        else rhs_at<double>(K, d, tt, R0, Pin0, s, o);
+    };
+    auto f32 = [&](double tt, const double* s, double* o) { rhs_at<float>(K, d, tt, R0, Pin0, s, o); };
     while (true) {
         if (!have) {
             if (next >= end) {
@@ -298,6 +400,10 @@ __global__ void __launch_bounds__(256) gm_batch_kernel(GmBatchArgs a) {
             Pin0 = K.m64.P0 + K.m64.S2 / R0;
 #pragma unroll
             for (int k = 0; k < 3; ++k) { d.w[k] = a.w[k]; d.A[k] = a.amp[3 * i + k]; d.ph[k] = a.phase[3 * i + k]; }
+            rd.mode = 3;
+            rd.A = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) { rd.f[k] = a.f[k]; rd.a[k] = d.A[k]; rd.ph[k] = d.ph[k]; }
             nsteps_rk = llrint(a.t_end / a.dt);
             if (a.resume) {
                 const GmLane& L = a.state[i];
@@ -318,8 +424,8 @@ __global__ void __launch_bounds__(256) gm_batch_kernel(GmBatchArgs a) {
         bool done = false;
         const bool use32 = PREC == BD_GM_SPLIT && y[0] >= a.split_ratio * R0;
         if constexpr (METHOD == BD_GM_RK4) {
-            if (PREC == BD_GM_SPLIT && use32) rk4_step<float>(K, d, R0, Pin0, s_rk * a.dt, a.dt, y);
-            else rk4_step<double>(K, d, R0, Pin0, s_rk * a.dt, a.dt, y);
+            if (PREC == BD_GM_SPLIT && use32) rk4_step(f32, s_rk * a.dt, a.dt, y);
+            else rk4_step(f64, s_rk * a.dt, a.dt, y);
             ++s_rk; ++acc;
             t = s_rk * a.dt;
             if (y[0] > rmax) rmax = y[0];
@@ -329,8 +435,8 @@ __global__ void __launch_bounds__(256) gm_batch_kernel(GmBatchArgs a) {
             if (s_rk >= nsteps_rk || (a.max_steps > 0 && acc >= a.max_steps)) done = true;
         } else {
             if (!k1_ok) {
-                if (PREC == BD_GM_SPLIT && use32) rhs_at<float>(K, d, t, R0, Pin0, y, k1);
-                else rhs_at<double>(K, d, t, R0, Pin0, y, k1);
+                if (PREC == BD_GM_SPLIT && use32) f32(t, y, k1);
+                else f64(t, y, k1);
                 k1_ok = true;
             }
             if (at_start) {
@@ -340,8 +446,8 @@ __global__ void __launch_bounds__(256) gm_batch_kernel(GmBatchArgs a) {
             }
             double Y[4], k7[4];
             double err;
-            if (PREC == BD_GM_SPLIT && use32) err = dp_attempt<float>(K, d, R0, Pin0, t, y, h, k1, Y, k7);
-            else err = dp_attempt<double>(K, d, R0, Pin0, t, y, h, k1, Y, k7);
+            if (PREC == BD_GM_SPLIT && use32) err = dp_attempt(K, f32, t, y, h, k1, Y, k7);
+            else err = dp_attempt(K, f64, t, y, h, k1, Y, k7);
             const double mr = step_factor(err);
             bool accept = !(err > 1.0);
             if (!accept) {
@@ -355,6 +461,8 @@ __global__ void __launch_bounds__(256) gm_batch_kernel(GmBatchArgs a) {
                 t = t + h;
 #pragma unroll
                 for (int c = 0; c < 4; ++c) { y[c] = Y[c]; k1[c] = k7[c]; }
+                // the reference recomputes the first stage at the new point (G:259-273)
+                if (ARITH == BD_GM_ARITH_REFERENCE) k1_ok = false;
                 h = h * fmin(fac_max, mr);
                 at_start = true;
                 ++acc;
@@ -392,15 +500,30 @@ __global__ void __launch_bounds__(256) gm_batch_kernel(GmBatchArgs a) {
 // ---------------------------------------------------------------------------
 // drop-in threshold sweep (G:323-429), persistent with per-lane refill
 // ---------------------------------------------------------------------------
+// Search state of a (combo, R0) cell suspended at the end of a launch.
+struct GmCell {
+    double y[4], k1[4];
+    double t, h, fac_max, A5, A4, A3;
+    long long steps;
+    int idx, stage, flags, pad;
+};
+constexpr int kCellK1 = 1, kCellStart = 2, kCellThr5 = 4, kCellThr4 = 8, kCellThr3 = 16;
+
 struct GmSweepArgs {
     GmConsts K;
     int triple, N_fr_combo, N_R0;
     double minR0, stepR0;
     double* results_A;
     const double* Fr_combo;
-    int* counter;              // [0] cells claimed, [1] cells completed (device)
+    // device counters: [0] cells claimed, [1] cells completed, [2] suspended
+    // cells taken up, [3 + b] cells suspended into buffer b (b = 0, 1)
+    int* counter;
     int cell0, cell_end;       // cells idx in [cell0, cell_end), claimed in increasing order
     int* host_flags;           // nullable, host-mapped: [0] claimed, [1] completed, [2] stop request
+    const GmCell* susp_in;     // cells to continue first (counter[3 + b_in] of them), nullable
+    GmCell* susp_out;          // cells this launch suspends (buffer 1 - b_in), nullable if budget = 0
+    int b_in;
+    long long budget;          // DP5(4) attempts per lane in this launch, 0 = no limit
 };
 
 // System-scope access to the host-mapped progress words (vector memory ops).
@@ -424,7 +547,17 @@ __device__ __forceinline__ void ref_drive(GmDrive& d, bool triple, const double 
     }
 }
 
-__global__ void __launch_bounds__(256) gm_sweep_kernel(GmSweepArgs a) {
+// The same drive in the reference's own form (G:40-78).
+__device__ __forceinline__ void ref_drive_mode(GmRefDrive& d, bool triple, const double fr[3], double A) {
+    d.A = A;
+    for (int k = 0; k < 3; ++k) { d.f[k] = fr[k]; d.a[k] = 0.0; d.ph[k] = 0.0; }
+    if (!triple || (fr[0] == fr[1] && fr[0] == fr[2])) d.mode = 0;
+    else if (fr[0] == fr[1] || fr[1] == fr[2]) d.mode = 1;
+    else d.mode = 2;
+}
+
+template <int ARITH>
+__global__ void __launch_bounds__(64) gm_sweep_kernel(GmSweepArgs a) {
     const GmConsts& K = a.K;
     const double A_MAX = 1.0e7;
     bool have = false;
@@ -434,55 +567,96 @@ __global__ void __launch_bounds__(256) gm_sweep_kernel(GmSweepArgs a) {
     double y[4] = {0, 0, 0, 0}, k1[4] = {0, 0, 0, 0};
     bool k1_ok = false, at_start = true;
     int stage = 0;          // 1, 2, 3: search stage of G:371-426
-    double A5 = 0, A4 = 0, A3 = 0, result = -1.0;
+    double A5 = 0, A4 = 0, A3 = 0;
     bool thr5 = false, thr4 = true, thr3 = true;
-    long long steps = 0;
+    long long steps = 0;    // accepted steps of the current solve
+    long long used = 0;     // attempts of this lane in this launch
     GmDrive d;
-
+    GmRefDrive rd;
+    auto rhs = [&](double tt, const double* s, double* o) {
+        if constexpr (ARITH == BD_GM_ARITH_REFERENCE) gm_rhs_ref(K.ref, rd, tt, R0, s, o);
+        else rhs_at<double>(K, d, tt, R0, Pin0, s, o);
+    };
+    auto set_amplitude = [&](double A) {
+        if constexpr (ARITH == BD_GM_ARITH_REFERENCE) ref_drive_mode(rd, a.triple != 0, fr, A);
+        else ref_drive(d, a.triple != 0, fr, A);
+    };
     auto start_solve = [&](double A, bool reset_h) {
         if (reset_h) h = K.h0;
         t = 0.0; y[0] = R0; y[1] = y[2] = y[3] = 0.0;
-        ref_drive(d, a.triple != 0, fr, A);
+        set_amplitude(A);
         k1_ok = false; at_start = true; steps = 0;
+    };
+    auto set_cell = [&](int id) {        // the cell's fixed inputs (G:332-353)
+        const int ic = id / a.N_R0, ir = id % a.N_R0;
+        R0 = a.minR0 + ir * a.stepR0;
+        Pin0 = K.m64.P0 + K.m64.S2 / R0;
+        fr[0] = a.Fr_combo[3 * ic]; fr[1] = a.Fr_combo[3 * ic + 1]; fr[2] = a.Fr_combo[3 * ic + 2];
+        if (a.triple) {
+            if (fr[0] <= fr[1] && fr[0] <= fr[2]) endT = 5.0 / fr[0];
+            if (fr[1] <= fr[0] && fr[1] <= fr[2]) endT = 5.0 / fr[1];
+            if (fr[2] <= fr[0] && fr[2] <= fr[1]) endT = 5.0 / fr[2];
+        } else {
+            endT = 5.0 / fr[0];
+        }
+    };
+    auto finish_cell = [&](double result) {
+        a.results_A[idx] = result;
+        have = false;
+        const int done = atomicAdd(a.counter + 1, 1) + 1;
+        if (a.host_flags && (done & 63) == 0) host_store(a.host_flags + 1, done);
     };
 
     while (true) {
         if (!have) {
-            // a stop request ends the claiming; every claimed cell is completed,
-            // so the completed cells are always a prefix of the range
+            if (a.budget > 0 && used >= a.budget) break;
+            // a stop request ends the claiming
             if (a.host_flags && host_load(a.host_flags + 2) != 0) break;
-            idx = a.cell0 + atomicAdd(a.counter, 1);
-            if (idx >= a.cell_end) break;
-            if (a.host_flags && ((idx - a.cell0) & 255) == 0) host_store(a.host_flags, idx - a.cell0 + 1);
-            have = true;
-            const int ic = idx / a.N_R0, ir = idx % a.N_R0;
-            R0 = a.minR0 + ir * a.stepR0;
-            Pin0 = K.m64.P0 + K.m64.S2 / R0;
-            fr[0] = a.Fr_combo[3 * ic]; fr[1] = a.Fr_combo[3 * ic + 1]; fr[2] = a.Fr_combo[3 * ic + 2];
-            if (a.triple) {  // G:342-353
-                if (fr[0] <= fr[1] && fr[0] <= fr[2]) endT = 5.0 / fr[0];
-                if (fr[1] <= fr[0] && fr[1] <= fr[2]) endT = 5.0 / fr[1];
-                if (fr[2] <= fr[0] && fr[2] <= fr[1]) endT = 5.0 / fr[2];
-            } else {
-                endT = 5.0 / fr[0];
+            if (a.susp_in) {
+                // continue the cells an earlier launch suspended first
+                const int n_in = a.counter[3 + a.b_in];
+                if (__hip_atomic_load(a.counter + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < n_in) {
+                    const int r = atomicAdd(a.counter + 2, 1);
+                    if (r < n_in) {
+                        const GmCell& c = a.susp_in[r];
+                        idx = c.idx;
+                        set_cell(idx);
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) { y[k] = c.y[k]; k1[k] = c.k1[k]; }
+                        t = c.t; h = c.h; fac_max = c.fac_max; A5 = c.A5; A4 = c.A4; A3 = c.A3;
+                        steps = c.steps; stage = c.stage;
+                        k1_ok = (c.flags & kCellK1) != 0; at_start = (c.flags & kCellStart) != 0;
+                        thr5 = (c.flags & kCellThr5) != 0; thr4 = (c.flags & kCellThr4) != 0;
+                        thr3 = (c.flags & kCellThr3) != 0;
+                        set_amplitude(stage == 1 ? A5 : (stage == 2 ? A4 : A3));
+                        have = true;
+                    }
+                }
             }
-            h = K.h0;
-            result = -1.0;
-            thr5 = false; thr4 = true; thr3 = true;
-            A5 = A4 = A3 = 0.0;
-            stage = 1;
-            A5 = A5 + 1.0e5;
-            start_solve(A5, false);
+            if (!have) {
+                idx = a.cell0 + atomicAdd(a.counter, 1);
+                if (idx >= a.cell_end) break;
+                if (a.host_flags && ((idx - a.cell0) & 255) == 0) host_store(a.host_flags, idx - a.cell0 + 1);
+                have = true;
+                set_cell(idx);
+                h = K.h0;
+                thr5 = false; thr4 = true; thr3 = true;
+                A5 = A4 = A3 = 0.0;
+                stage = 1;
+                A5 = A5 + 1.0e5;
+                start_solve(A5, false);
+            }
         }
         // one DP5(4) attempt of the current solve
-        if (!k1_ok) { rhs_at<double>(K, d, t, R0, Pin0, y, k1); k1_ok = true; }
+        if (!k1_ok) { rhs(t, y, k1); k1_ok = true; }
         if (at_start) {
             if (t + h > endT) h = endT - t;
             fac_max = 5.0;
             at_start = false;
         }
         double Y[4], k7[4];
-        const double err = dp_attempt<double>(K, d, R0, Pin0, t, y, h, k1, Y, k7);
+        const double err = dp_attempt(K, rhs, t, y, h, k1, Y, k7);
+        ++used;
         const double mr = step_factor(err);
         bool accept = !(err > 1.0);
         if (!accept) {
@@ -490,41 +664,67 @@ __global__ void __launch_bounds__(256) gm_sweep_kernel(GmSweepArgs a) {
             h = h * fmin(fac_max, mr);
             if (!(h > 0.0) || !isfinite(err)) accept = true;
         }
-        if (!accept) continue;
-        t = t + h;
+        bool solve_over = false, thr = false;
+        if (accept) {
+            t = t + h;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) { y[c] = Y[c]; k1[c] = k7[c]; }
-        h = h * fmin(fac_max, mr);
-        at_start = true;
-        ++steps;
-        const bool thr = Y[1] <= -K.C_inf;
-        if (t < endT && !thr && steps < K.cap) continue;
-        // the solve is over: advance the search (G:372-426)
-        if (stage == 1) {
-            thr5 = thr;
-            if (!thr5 && A5 < A_MAX) { A5 = A5 + 1.0e5; start_solve(A5, false); continue; }
-            if (!thr5 && A5 == A_MAX) result = A5;
-            A4 = A5;
-            if (thr5 && thr4) { stage = 2; A4 = A4 - 1.0e4; thr4 = false; start_solve(A4, true); continue; }
-            stage = 3;
-        } else if (stage == 2) {
-            thr4 = thr;
-            if (thr4) { A4 = A4 - 1.0e4; thr4 = false; start_solve(A4, true); continue; }
-            A4 = A4 + 1.0e4;
-            A3 = A4;
-            if (thr5 && !thr4 && thr3) { stage = 3; A3 = A3 - 1.0e3; thr3 = false; start_solve(A3, true); continue; }
-        } else {
-            thr3 = thr;
-            if (!thr3) {
-                result = A3 + 1.0e3;
-            } else {
-                A3 = A3 - 1.0e3; thr3 = false; start_solve(A3, true); continue;
-            }
+            for (int c = 0; c < 4; ++c) { y[c] = Y[c]; k1[c] = k7[c]; }
+            // the reference recomputes the first stage at the new point (G:259-273)
+            if (ARITH == BD_GM_ARITH_REFERENCE) k1_ok = false;
+            h = h * fmin(fac_max, mr);
+            at_start = true;
+            ++steps;
+            thr = Y[1] <= -K.C_inf;
+            solve_over = !(t < endT && !thr && steps < K.cap);
         }
-        a.results_A[idx] = result;
-        have = false;
-        const int done = atomicAdd(a.counter + 1, 1) + 1;
-        if (a.host_flags && (done & 63) == 0) host_store(a.host_flags + 1, done);
+        if (solve_over) {
+            if (!thr && t < endT) {
+                // the solve reached the step cap undecided: the search cannot go on
+                finish_cell(BD_GM_SWEEP_CAPPED);
+                continue;
+            }
+            // advance the search (G:372-426)
+            bool next_solve = false;
+            double result = -1.0;
+            if (stage == 1) {
+                thr5 = thr;
+                if (!thr5 && A5 < A_MAX) { A5 = A5 + 1.0e5; start_solve(A5, false); next_solve = true; }
+                else {
+                    if (!thr5 && A5 == A_MAX) result = A5;
+                    A4 = A5;
+                    if (thr5 && thr4) { stage = 2; A4 = A4 - 1.0e4; thr4 = false; start_solve(A4, true); next_solve = true; }
+                    else stage = 3;
+                }
+            } else if (stage == 2) {
+                thr4 = thr;
+                if (thr4) { A4 = A4 - 1.0e4; thr4 = false; start_solve(A4, true); next_solve = true; }
+                else {
+                    A4 = A4 + 1.0e4;
+                    A3 = A4;
+                    if (thr5 && !thr4 && thr3) { stage = 3; A3 = A3 - 1.0e3; thr3 = false; start_solve(A3, true); next_solve = true; }
+                }
+            } else {
+                thr3 = thr;
+                if (!thr3) result = A3 + 1.0e3;
+                else { A3 = A3 - 1.0e3; thr3 = false; start_solve(A3, true); next_solve = true; }
+            }
+            if (!next_solve) { finish_cell(result); continue; }
+        }
+        // out of budget, or asked to stop: suspend the cell for a later launch
+        const bool stop = a.host_flags && (used & 255) == 0 && host_load(a.host_flags + 2) != 0;
+        if (a.susp_out && ((a.budget > 0 && used >= a.budget) || stop)) {
+            GmCell c;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) { c.y[k] = y[k]; c.k1[k] = k1[k]; }
+            c.t = t; c.h = h; c.fac_max = fac_max; c.A5 = A5; c.A4 = A4; c.A3 = A3;
+            c.steps = steps; c.idx = idx; c.stage = stage; c.pad = 0;
+            c.flags = (k1_ok ? kCellK1 : 0) | (at_start ? kCellStart : 0) | (thr5 ? kCellThr5 : 0) |
+                      (thr4 ? kCellThr4 : 0) | (thr3 ? kCellThr3 : 0);
+            const int pos = atomicAdd(a.counter + 3 + (1 - a.b_in), 1);
+            a.susp_out[pos] = c;
+            have = false;
+            if (stop) break;
+        }
     }
 }
 
@@ -536,6 +736,25 @@ __global__ void gm_sweep_final_kernel(const int* counter, int* host_flags) {
     }
 }
 
+// After an advance of a resumable sweep: claimed (clamped to the range),
+// completed, suspended by this launch, then the finished mark.
+__global__ void gm_sweep_advance_final_kernel(const int* counter, int b_out, int range, int* host_flags) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        host_store(host_flags + 0, min(counter[0], range));
+        host_store(host_flags + 1, counter[1]);
+        host_store(host_flags + 4, counter[3 + b_out]);
+        host_store(host_flags + 3, 1);
+    }
+}
+
+// Before an advance: the resume cursor and the output buffer's count start at 0.
+__global__ void gm_sweep_advance_init_kernel(int* counter, int b_out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        counter[2] = 0;
+        counter[3 + b_out] = 0;
+    }
+}
+
 // zeroes p[0 .. blockDim.x) (a fill kernel: safe inside captured graphs)
 __global__ void zero_int_kernel(int* p) { if (blockIdx.x == 0) p[threadIdx.x] = 0; }
 
@@ -543,6 +762,7 @@ static GmConsts make_consts(const bd_gilmore_params* g) {
     GmConsts K;
     K.m64 = make_model<double>(g);
     K.m32 = make_model<float>(g);
+    K.ref = make_ref_model(g);
     K.tab = make_tab(g->tableau);
     K.C_inf = g->C_inf;
     K.abs_eps = g->abs_eps;
@@ -588,13 +808,14 @@ static GmShape gm_shape(const bd_gilmore_batch* b) {
     return s;
 }
 
-// two counters per device for the drop-in sweep (its signature has no workspace)
+// the drop-in sweep's counters, one set per device (its signature has no workspace)
+constexpr int kSweepCounters = 5;
 static int* sweep_counter() {
     static int* ctr[64] = {nullptr};
     std::lock_guard<std::mutex> lock(g_gm_mutex);
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
-    if (!ctr[dev] && hipMalloc(&ctr[dev], 2 * sizeof(int)) != hipSuccess) ctr[dev] = nullptr;
+    if (!ctr[dev] && hipMalloc(&ctr[dev], kSweepCounters * sizeof(int)) != hipSuccess) ctr[dev] = nullptr;
     return ctr[dev];
 }
 
@@ -646,6 +867,7 @@ extern "C" void bd_gilmore_reference_params(bd_gilmore_params* p) {  // G:440-46
     p->rel_eps = std::pow(10.0, -9);
     p->h0 = std::pow(10.0, -13);
     p->max_solve_steps = 1e7;
+    p->arithmetic = BD_GM_ARITH_REFERENCE;
 }
 
 static int check_gilmore(const bd_gilmore_params* g) {
@@ -653,65 +875,19 @@ static int check_gilmore(const bd_gilmore_params* g) {
     if (!(g->rho > 0) || !(g->lambda > 0) || !(g->n > 1) || !(g->C_inf > 0) || !(g->h0 > 0))
         return BD_ERR_INVALID;
     if (g->tableau != BD_GM_TAB_REFERENCE && g->tableau != BD_GM_TAB_DP1980) return BD_ERR_INVALID;
+    if (g->arithmetic != BD_GM_ARITH_REFERENCE && g->arithmetic != BD_GM_ARITH_FAST) return BD_ERR_INVALID;
     return BD_OK;
 }
 
-static int launch_sweep(GmSweepArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(2), 0, st, a.counter);
-    BD_LAUNCH_CHECK();
-    const long long total = (long long)a.cell_end - a.cell0;
-    const int block = 64;
-    int blocks = persistent_blocks(block);
-    blocks = (int)std::min<long long>(blocks, (total + block - 1) / block);
-    hipLaunchKernelGGL(gm_sweep_kernel, dim3(blocks), dim3(block), 0, st, a);
-    BD_LAUNCH_CHECK();
-    return BD_OK;
+// Launch geometry of the sweep kernel: 64-lane workgroups, 8 waves per SIMD.
+constexpr int kSweepBlock = 64;
+static int sweep_blocks(long long cells) {
+    const int blocks = persistent_blocks(kSweepBlock);
+    return (int)std::max<long long>(1, std::min<long long>(blocks, (cells + kSweepBlock - 1) / kSweepBlock));
 }
 
-extern "C" int bd_gilmore_threshold_sweep(const bd_gilmore_params* prm, double* results_A,
-                                          const double* Fr_combo, int N_fr_combo, int N_R0, int N_A,
-                                          void* stream) {
-    (void)N_A;
-    int rc = check_gilmore(prm);
-    if (rc) return rc;
-    if (!results_A || !Fr_combo || N_fr_combo <= 0 || N_R0 <= 0) return BD_ERR_INVALID;
-    if ((long long)N_fr_combo * N_R0 > 0x7fffffffLL) return BD_ERR_INVALID;
-    int* ctr = sweep_counter();
-    if (!ctr) return BD_ERR_ALLOC;
-    hipStream_t st = as_stream(stream);
-    GmSweepArgs a;
-    a.K = make_consts(prm);
-    a.triple = prm->triple_fr;
-    a.N_fr_combo = N_fr_combo;
-    a.N_R0 = N_R0;
-    a.minR0 = prm->minR0;
-    a.stepR0 = prm->stepR0;
-    a.results_A = results_A;
-    a.Fr_combo = Fr_combo;
-    a.counter = ctr;
-    a.cell0 = 0;
-    a.cell_end = N_fr_combo * N_R0;
-    a.host_flags = nullptr;
-    return launch_sweep(a, st);
-}
-
-extern "C" int bd_gilmore_threshold_sweep_range(const bd_gilmore_params* prm, double* results_A,
-                                                const double* Fr_combo, int N_fr_combo, int N_R0, int cell0,
-                                                int cell_end, void* stream) {
-    int rc = check_gilmore(prm);
-    if (rc) return rc;
-    if (!results_A || !Fr_combo || N_fr_combo <= 0 || N_R0 <= 0) return BD_ERR_INVALID;
-    if ((long long)N_fr_combo * N_R0 > 0x7fffffffLL) return BD_ERR_INVALID;
-    if (cell0 < 0 || cell_end > N_fr_combo * N_R0 || cell0 >= cell_end) return BD_ERR_INVALID;
-    int* ctr = sweep_counter();
-    if (!ctr) return BD_ERR_ALLOC;
-    int *hf = nullptr, *df = nullptr;
-    if (!sweep_flags(&hf, &df)) return BD_ERR_ALLOC;
-    hipStream_t st = as_stream(stream);
-    // the caller's stop request (hf[2]) is left as it is: a stop set before the
-    // call makes the launch claim nothing
-    hf[0] = 0; hf[1] = 0; hf[3] = 0;
-    GmSweepArgs a;
+static void sweep_args(GmSweepArgs& a, const bd_gilmore_params* prm, double* results_A, const double* Fr_combo,
+                       int N_fr_combo, int N_R0, int* ctr, int cell0, int cell_end) {
     a.K = make_consts(prm);
     a.triple = prm->triple_fr;
     a.N_fr_combo = N_fr_combo;
@@ -723,13 +899,177 @@ extern "C" int bd_gilmore_threshold_sweep_range(const bd_gilmore_params* prm, do
     a.counter = ctr;
     a.cell0 = cell0;
     a.cell_end = cell_end;
+    a.host_flags = nullptr;
+    a.susp_in = nullptr;
+    a.susp_out = nullptr;
+    a.b_in = 0;
+    a.budget = 0;
+}
+
+static int launch_sweep_kernel(const GmSweepArgs& a, int arith, int blocks, hipStream_t st) {
+    if (arith == BD_GM_ARITH_REFERENCE)
+        hipLaunchKernelGGL(gm_sweep_kernel<BD_GM_ARITH_REFERENCE>, dim3(blocks), dim3(kSweepBlock), 0, st, a);
+    else
+        hipLaunchKernelGGL(gm_sweep_kernel<BD_GM_ARITH_FAST>, dim3(blocks), dim3(kSweepBlock), 0, st, a);
+    BD_LAUNCH_CHECK();
+    return BD_OK;
+}
+
+// One unbudgeted launch over [cell0, cell_end) from zeroed counters.
+static int launch_sweep(GmSweepArgs& a, int arith, hipStream_t st) {
+    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(kSweepCounters), 0, st, a.counter);
+    BD_LAUNCH_CHECK();
+    return launch_sweep_kernel(a, arith, sweep_blocks((long long)a.cell_end - a.cell0), st);
+}
+
+static int check_sweep(const bd_gilmore_params* prm, const double* results_A, const double* Fr_combo,
+                       int N_fr_combo, int N_R0) {
+    int rc = check_gilmore(prm);
+    if (rc) return rc;
+    if (!results_A || !Fr_combo || N_fr_combo <= 0 || N_R0 <= 0) return BD_ERR_INVALID;
+    if ((long long)N_fr_combo * N_R0 > 0x7fffffffLL) return BD_ERR_INVALID;
+    return BD_OK;
+}
+
+extern "C" int bd_gilmore_threshold_sweep(const bd_gilmore_params* prm, double* results_A,
+                                          const double* Fr_combo, int N_fr_combo, int N_R0, int N_A,
+                                          void* stream) {
+    (void)N_A;
+    int rc = check_sweep(prm, results_A, Fr_combo, N_fr_combo, N_R0);
+    if (rc) return rc;
+    int* ctr = sweep_counter();
+    if (!ctr) return BD_ERR_ALLOC;
+    GmSweepArgs a;
+    sweep_args(a, prm, results_A, Fr_combo, N_fr_combo, N_R0, ctr, 0, N_fr_combo * N_R0);
+    return launch_sweep(a, prm->arithmetic, as_stream(stream));
+}
+
+extern "C" int bd_gilmore_threshold_sweep_range(const bd_gilmore_params* prm, double* results_A,
+                                                const double* Fr_combo, int N_fr_combo, int N_R0, int cell0,
+                                                int cell_end, void* stream) {
+    int rc = check_sweep(prm, results_A, Fr_combo, N_fr_combo, N_R0);
+    if (rc) return rc;
+    if (cell0 < 0 || cell_end > N_fr_combo * N_R0 || cell0 >= cell_end) return BD_ERR_INVALID;
+    int* ctr = sweep_counter();
+    if (!ctr) return BD_ERR_ALLOC;
+    int *hf = nullptr, *df = nullptr;
+    if (!sweep_flags(&hf, &df)) return BD_ERR_ALLOC;
+    hipStream_t st = as_stream(stream);
+    // the caller's stop request (hf[2]) is left as it is: a stop set before the
+    // call makes the launch claim nothing
+    hf[0] = 0; hf[1] = 0; hf[3] = 0;
+    GmSweepArgs a;
+    sweep_args(a, prm, results_A, Fr_combo, N_fr_combo, N_R0, ctr, cell0, cell_end);
     a.host_flags = df;
-    rc = launch_sweep(a, st);
+    rc = launch_sweep(a, prm->arithmetic, st);
     if (rc) return rc;
     hipLaunchKernelGGL(gm_sweep_final_kernel, dim3(1), dim3(1), 0, st, (const int*)ctr, df);
     BD_LAUNCH_CHECK();
     return BD_OK;
 }
+
+// Resumable sweep: the drop-in's kernel in launches of bounded length.
+struct bd_gilmore_sweep {
+    bd::GmSweepArgs base;
+    int arith = 0;
+    int device = 0;
+    int blocks = 0;            // persistent grid of every advance (suspension capacity = lanes)
+    int* d_ctr = nullptr;      // kSweepCounters words
+    bd::GmCell* susp[2] = {nullptr, nullptr};
+    int b_in = 0;              // buffer holding the cells the next advance continues
+    int* h_flags = nullptr;    // host-mapped: [0] claimed [1] completed [2] stop [3] launch ended [4] suspended
+    int* d_flags = nullptr;
+};
+
+extern "C" int bd_gilmore_sweep_destroy(bd_gilmore_sweep* sw) {
+    if (!sw) return BD_OK;
+    (void)hipDeviceSynchronize();
+    if (sw->d_ctr) (void)hipFree(sw->d_ctr);
+    for (int b = 0; b < 2; ++b) if (sw->susp[b]) (void)hipFree(sw->susp[b]);
+    if (sw->h_flags) (void)hipHostFree(sw->h_flags);
+    delete sw;
+    return BD_OK;
+}
+
+extern "C" int bd_gilmore_sweep_create(const bd_gilmore_params* prm, const double* Fr_combo, int N_fr_combo,
+                                       int N_R0, int cell0, int cell_end, bd_gilmore_sweep** out) {
+    using namespace bd;
+    if (!out) return BD_ERR_INVALID;
+    *out = nullptr;
+    int rc = check_gilmore(prm);
+    if (rc) return rc;
+    if (!Fr_combo || N_fr_combo <= 0 || N_R0 <= 0 || (long long)N_fr_combo * N_R0 > 0x7fffffffLL)
+        return BD_ERR_INVALID;
+    if (cell0 < 0 || cell_end > N_fr_combo * N_R0 || cell0 >= cell_end) return BD_ERR_INVALID;
+    auto* sw = new bd_gilmore_sweep();
+    sw->arith = prm->arithmetic;
+    sweep_args(sw->base, prm, nullptr, Fr_combo, N_fr_combo, N_R0, nullptr, cell0, cell_end);
+    sw->blocks = sweep_blocks((long long)cell_end - cell0);
+    const size_t cap = (size_t)sw->blocks * kSweepBlock;
+    void* hp = nullptr;
+    void* dp = nullptr;
+    if (hipGetDevice(&sw->device) != hipSuccess ||
+        hipMalloc(&sw->d_ctr, kSweepCounters * sizeof(int)) != hipSuccess ||
+        hipMalloc(&sw->susp[0], cap * sizeof(GmCell)) != hipSuccess ||
+        hipMalloc(&sw->susp[1], cap * sizeof(GmCell)) != hipSuccess ||
+        hipHostMalloc(&hp, 8 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+        bd_gilmore_sweep_destroy(sw);
+        return BD_ERR_ALLOC;
+    }
+    sw->h_flags = (int*)hp;
+    std::memset(hp, 0, 8 * sizeof(int));
+    if (hipHostGetDevicePointer(&dp, hp, 0) != hipSuccess ||
+        hipMemset(sw->d_ctr, 0, kSweepCounters * sizeof(int)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        bd_gilmore_sweep_destroy(sw);
+        return BD_ERR_HIP;
+    }
+    sw->d_flags = (int*)dp;
+    sw->h_flags[3] = 1;
+    *out = sw;
+    return BD_OK;
+}
+
+extern "C" int bd_gilmore_sweep_advance(bd_gilmore_sweep* sw, double* results_A, long long budget, void* stream) {
+    using namespace bd;
+    if (!sw || !results_A || budget < 0) return BD_ERR_INVALID;
+    int dev = 0;
+    BD_HIP_CHECK(hipGetDevice(&dev));
+    if (dev != sw->device) return BD_ERR_INVALID;
+    hipStream_t st = as_stream(stream);
+    const int b_out = 1 - sw->b_in;
+    GmSweepArgs a = sw->base;
+    a.results_A = results_A;
+    a.counter = sw->d_ctr;
+    a.host_flags = sw->d_flags;
+    a.susp_in = sw->susp[sw->b_in];
+    a.susp_out = sw->susp[b_out];
+    a.b_in = sw->b_in;
+    a.budget = budget;
+    sw->h_flags[3] = 0;
+    hipLaunchKernelGGL(gm_sweep_advance_init_kernel, dim3(1), dim3(1), 0, st, sw->d_ctr, b_out);
+    BD_LAUNCH_CHECK();
+    int rc = launch_sweep_kernel(a, sw->arith, sw->blocks, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(gm_sweep_advance_final_kernel, dim3(1), dim3(1), 0, st, (const int*)sw->d_ctr, b_out,
+                       a.cell_end - a.cell0, sw->d_flags);
+    BD_LAUNCH_CHECK();
+    sw->b_in = b_out;
+    return BD_OK;
+}
+
+extern "C" int bd_gilmore_sweep_progress(const bd_gilmore_sweep* sw, long long out[5]) {
+    if (!sw || !out) return BD_ERR_INVALID;
+    const volatile int* f = sw->h_flags;
+    const int range = sw->base.cell_end - sw->base.cell0;
+    out[0] = f[0];
+    out[1] = f[1];
+    out[2] = f[4];
+    out[3] = f[1] >= range ? 1 : 0;
+    out[4] = f[3];
+    return BD_OK;
+}
+
+extern "C" int* bd_gilmore_sweep_stop_word(bd_gilmore_sweep* sw) { return sw ? sw->h_flags + 2 : nullptr; }
 
 extern "C" int* bd_gilmore_sweep_flags(void) {
     int *hf = nullptr, *df = nullptr;
@@ -811,7 +1151,7 @@ static int gm_launch(const bd_gilmore_params* prm, const bd_gilmore_batch* b, vo
     a.out_every = b->out_every;
     a.n_out = b->R_out ? b->n_out : 0;
     a.chunk = shp.chunk;
-    for (int k = 0; k < 3; ++k) a.w[k] = 2.0 * M_PI * b->freq[k];
+    for (int k = 0; k < 3; ++k) { a.w[k] = 2.0 * M_PI * b->freq[k]; a.f[k] = b->freq[k]; }
     a.t_end = b->t_end;
     a.dt = b->dt;
     a.split_ratio = b->split_ratio;
@@ -832,14 +1172,20 @@ static int gm_launch(const bd_gilmore_params* prm, const bd_gilmore_batch* b, vo
     const long long need = ((long long)b->n + (long long)a.chunk - 1) / a.chunk;
     blocks = (int)std::min<long long>(blocks, (need + block - 1) / block);
     const dim3 grid(std::max(1, blocks)), blk(block);
-    if (b->method == BD_GM_RK4 && b->precision == BD_GM_FP64)
-        hipLaunchKernelGGL((gm_batch_kernel<BD_GM_RK4, BD_GM_FP64>), grid, blk, 0, st, a);
+    constexpr int REF = BD_GM_ARITH_REFERENCE, FAST = BD_GM_ARITH_FAST;
+    const bool ref = prm->arithmetic == REF;
+    if (b->method == BD_GM_RK4 && b->precision == BD_GM_FP64 && ref)
+        hipLaunchKernelGGL((gm_batch_kernel<BD_GM_RK4, BD_GM_FP64, REF>), grid, blk, 0, st, a);
+    else if (b->method == BD_GM_RK4 && b->precision == BD_GM_FP64)
+        hipLaunchKernelGGL((gm_batch_kernel<BD_GM_RK4, BD_GM_FP64, FAST>), grid, blk, 0, st, a);
     else if (b->method == BD_GM_RK4)
-        hipLaunchKernelGGL((gm_batch_kernel<BD_GM_RK4, BD_GM_SPLIT>), grid, blk, 0, st, a);
+        hipLaunchKernelGGL((gm_batch_kernel<BD_GM_RK4, BD_GM_SPLIT, FAST>), grid, blk, 0, st, a);
+    else if (b->precision == BD_GM_FP64 && ref)
+        hipLaunchKernelGGL((gm_batch_kernel<BD_GM_DP45, BD_GM_FP64, REF>), grid, blk, 0, st, a);
     else if (b->precision == BD_GM_FP64)
-        hipLaunchKernelGGL((gm_batch_kernel<BD_GM_DP45, BD_GM_FP64>), grid, blk, 0, st, a);
+        hipLaunchKernelGGL((gm_batch_kernel<BD_GM_DP45, BD_GM_FP64, FAST>), grid, blk, 0, st, a);
     else
-        hipLaunchKernelGGL((gm_batch_kernel<BD_GM_DP45, BD_GM_SPLIT>), grid, blk, 0, st, a);
+        hipLaunchKernelGGL((gm_batch_kernel<BD_GM_DP45, BD_GM_SPLIT, FAST>), grid, blk, 0, st, a);
     BD_LAUNCH_CHECK();
     return BD_OK;
 }
@@ -851,6 +1197,8 @@ static int check_batch(const bd_gilmore_params* prm, const bd_gilmore_batch* b) 
         return BD_ERR_INVALID;
     if (b->method != BD_GM_RK4 && b->method != BD_GM_DP45) return BD_ERR_INVALID;
     if (b->precision != BD_GM_FP64 && b->precision != BD_GM_SPLIT) return BD_ERR_INVALID;
+    // the reference's arithmetic is FP64 throughout
+    if (b->precision == BD_GM_SPLIT && prm->arithmetic == BD_GM_ARITH_REFERENCE) return BD_ERR_UNSUPPORTED;
     if (!(b->dt > 0) || !(b->t_end > 0)) return BD_ERR_INVALID;
     if (b->R_out && b->n_out <= 0) return BD_ERR_INVALID;
     if (b->method == BD_GM_DP45 && (!(b->rtol > 0) || !(b->atol >= 0))) return BD_ERR_INVALID;

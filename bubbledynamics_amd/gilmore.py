@@ -13,22 +13,25 @@ import numpy as np
 import torch
 
 from . import _ffi
-from ._ffi import (BD_GM_DP45, BD_GM_FP64, BD_GM_RK4, BD_GM_SCHED_INPUT, BD_GM_SCHED_SLICED,  # noqa: F401
-                   BD_GM_SPLIT, BD_GM_TAB_DP1980, BD_GM_TAB_REFERENCE, GilmoreBatch, GilmoreParams, call)
+from ._ffi import (BD_GM_ARITH_FAST, BD_GM_ARITH_REFERENCE, BD_GM_DP45, BD_GM_FP64, BD_GM_RK4,  # noqa: F401
+                   BD_GM_SCHED_INPUT, BD_GM_SCHED_SLICED, BD_GM_SPLIT, BD_GM_SWEEP_CAPPED, BD_GM_TAB_DP1980,
+                   BD_GM_TAB_REFERENCE, GilmoreBatch, GilmoreParams, call)
 
 
 def reference_params() -> GilmoreParams:
-    """Liver constants of G:440-460, reference tableau and controller."""
+    """Liver constants of G:440-460, reference tableau, controller and arithmetic."""
     p = GilmoreParams()
     _ffi.load().bd_gilmore_reference_params(C.byref(p))
     return p
 
 
-def workload_params(published_tableau: bool = True) -> GilmoreParams:
+def workload_params(published_tableau: bool = True, arithmetic: int = BD_GM_ARITH_FAST) -> GilmoreParams:
     """Medium of the batched workloads: rho=1060, c=1590, sigma=0.056,
     mu=0.005, B=3.3e8, n=7, kappa=1.4; P0, G and lambda from the reference
-    (G:441, G:444, G:446)."""
+    (G:441, G:444, G:446).  The fast arithmetic by default; pass
+    BD_GM_ARITH_REFERENCE for the reference's expression order."""
     p = reference_params()
+    p.arithmetic = arithmetic
     p.rho = 1060.0
     p.C_inf = 1590.0
     p.S2 = 2 * 0.056
@@ -117,6 +120,64 @@ def threshold_sweep_range(params: GilmoreParams, fr_combo: np.ndarray, n_r0: int
     flags[2] = 0
     out[cell0:cell0 + done] = res[cell0:cell0 + done].cpu().numpy()
     return out, cell0 + done, time.perf_counter() - t0
+
+
+class ThresholdSweep:
+    """Resumable run of the drop-in sweep over cells [cell0, cell_end)
+    (bd_gilmore_sweep_*): each advance() is one launch in which every lane
+    stops after `budget` DP5(4) attempts, saving unfinished cells for the
+    next advance.  results (device, float64, NaN until a cell is done) is
+    indexed like the drop-in's output."""
+
+    def __init__(self, params: GilmoreParams, fr_combo: np.ndarray, n_r0: int, cell0: int = 0,
+                 cell_end: int | None = None):
+        fr_np = np.ascontiguousarray(fr_combo, dtype=np.float64).reshape(-1)
+        self.n_combo = fr_np.size // 3
+        self.total = self.n_combo * n_r0
+        self.cell0 = cell0
+        self.cell_end = self.total if cell_end is None else cell_end
+        self.fr = torch.as_tensor(fr_np, device="cuda")
+        self.results = torch.full((self.total,), float("nan"), dtype=torch.float64, device="cuda")
+        self._h = C.c_void_p()
+        call("bd_gilmore_sweep_create", C.byref(params), _vp(self.fr), self.n_combo, n_r0, cell0, self.cell_end,
+             C.byref(self._h))
+        self.advances = 0
+
+    def advance(self, budget: int) -> None:
+        call("bd_gilmore_sweep_advance", self._h, _vp(self.results), int(budget), _stream())
+        self.advances += 1
+
+    def progress(self) -> dict:
+        out = (C.c_longlong * 5)()
+        call("bd_gilmore_sweep_progress", self._h, out)
+        return {"claimed": out[0], "completed": out[1], "suspended": out[2], "all_done": bool(out[3]),
+                "launch_ended": bool(out[4])}
+
+    def stop_word(self):
+        return _ffi.load().bd_gilmore_sweep_stop_word(self._h)
+
+    def run(self, budget: int, max_advances: int = 1_000_000, log=None) -> np.ndarray:
+        """Advances until every cell of the range is done; returns the results."""
+        while self.advances < max_advances:
+            self.advance(budget)
+            torch.cuda.synchronize()
+            pr = self.progress()
+            if log:
+                log(f"sweep advance {self.advances}: {pr}")
+            if pr["all_done"]:
+                break
+        return self.results.cpu().numpy()
+
+    def close(self) -> None:
+        if self._h:
+            _ffi.load().bd_gilmore_sweep_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def write_sweep_table(path: str, fr_combo: np.ndarray, results: np.ndarray, n_r0: int, min_r0: float,

@@ -297,23 +297,43 @@ typedef struct bd_gilmore_params {
     double minA, stepA, minR0, stepR0;                                   /* G:29-33 */
     double abs_eps, rel_eps;   /* DP5(4) tolerances, G:214-215 (1e-9, 1e-9)       */
     double h0;                 /* first step of every solve, G:355 / G:393 (1e-13) */
-    double max_solve_steps;    /* cap on attempted steps of one ODE solve (0 = 1e7); the
+    double max_solve_steps;    /* cap on accepted steps of one ODE solve (0 = 1e7); the
                                   reference has none and can spin forever once h underflows */
-    double reserved[7];
+    int32_t arithmetic;        /* BD_GM_ARITH_REFERENCE or BD_GM_ARITH_FAST                    */
+    int32_t pad0;
+    double reserved[6];
 } bd_gilmore_params;
 
 #define BD_GM_TAB_REFERENCE 0
 #define BD_GM_TAB_DP1980    1
+
+/* Arithmetic of the right-hand side and of the first Runge-Kutta stage.
+ * REFERENCE evaluates G:40-179 in the reference's expression order (the
+ * cancelling forms pow(I, (n-1)/n) - 1 and (p_in - 2s/R + tau + B)/(P0 + p + B)
+ * included) and recomputes the first stage of every step (G:259-273), so
+ * results follow the reference to the last bits its libm leaves open; it is
+ * the default of bd_gilmore_reference_params.  FAST shares subexpressions,
+ * forms I^((n-1)/n) - 1 without cancellation and reuses the last stage of a
+ * step as the next step's first (FSAL): 6 instead of 7 right-hand sides per
+ * attempt, about 3x faster, equal in exact arithmetic and not bit-identical. */
+#define BD_GM_ARITH_REFERENCE 0
+#define BD_GM_ARITH_FAST      1
+
+/* Threshold-sweep result of a cell whose search met an ODE solve that ended
+ * neither in collapse nor at endT within max_solve_steps accepted steps (the
+ * reference, which has no cap, would have kept integrating that solve). */
+#define BD_GM_SWEEP_CAPPED (-2.0)
 
 void bd_gilmore_reference_params(bd_gilmore_params* p);
 
 /* Drop-in for DPRK_solver_threshold_fix_tissue<<<>>> (G:323-429): one result
  * per (frequency combo, R0) pair, idx = i_combo*N_R0 + i_R0; the inertial-
  * cavitation threshold amplitude found in 0.1 / 0.01 / 0.001 MPa stages, -1
- * if none is set, 1e7 if no collapse up to 10 MPa.  Fr_combo: 3*N_fr_combo
- * frequencies (device).  N_A is accepted and ignored, as in the reference.
- * The queue counter is one per device: sweeps that overlap in time on one
- * device (other streams or host threads) race; run them one after another. */
+ * if none is set, 1e7 if no collapse up to 10 MPa, BD_GM_SWEEP_CAPPED if a
+ * solve reached max_solve_steps.  Fr_combo: 3*N_fr_combo frequencies
+ * (device).  N_A is accepted and ignored, as in the reference.  The queue
+ * counter is one per device: sweeps that overlap in time on one device (other
+ * streams or host threads) race; run them one after another. */
 int bd_gilmore_threshold_sweep(const bd_gilmore_params* prm, double* results_A,
                                const double* Fr_combo, int N_fr_combo, int N_R0, int N_A,
                                void* stream);
@@ -368,6 +388,35 @@ int bd_gilmore_threshold_sweep_range(const bd_gilmore_params* prm, double* resul
  * (4 ints, allocated on first use, kept until the process ends); NULL on
  * allocation failure. */
 int* bd_gilmore_sweep_flags(void);
+
+/* ---- resumable threshold sweep (no counterpart in the reference) --------
+ * The drop-in sweep over cells [cell0, cell_end) in launches of bounded
+ * length, for tables whose cells take hours (the reference tableau at low
+ * frequency).  Each bd_gilmore_sweep_advance is one persistent launch in
+ * which every lane stops after `budget` DP5(4) attempts (0 = no limit) or
+ * within 256 attempts of a stop request: a lane holding an unfinished cell
+ * saves its whole search state (stage, amplitudes, thresholds, ODE state,
+ * first stage, step size, controller, step count) and the next advance
+ * continues those cells before it claims new ones.  Results equal the
+ * unsliced drop-in's bit for bit.  The handle owns its device state (about
+ * 150 bytes per lane of the persistent grid) and host-mapped progress words. */
+typedef struct bd_gilmore_sweep bd_gilmore_sweep;
+/* Fr_combo (device, 3*N_fr_combo doubles) must stay valid until destroy. */
+int bd_gilmore_sweep_create(const bd_gilmore_params* prm, const double* Fr_combo, int N_fr_combo, int N_R0,
+                            int cell0, int cell_end, bd_gilmore_sweep** out);
+/* One launch on the stream (asynchronous).  results_A (device) has
+ * N_fr_combo*N_R0 doubles indexed like the drop-in's; a cell's entry is
+ * written once, when its search ends. */
+int bd_gilmore_sweep_advance(bd_gilmore_sweep* sw, double* results_A, long long budget, void* stream);
+/* Progress after the last advance (read from host-mapped words; exact once
+ * out[4] is 1): out[0] cells claimed, out[1] cells completed, out[2] cells
+ * suspended (continued by the next advance), out[3] 1 when every cell of the
+ * range is completed, out[4] 1 when the last advance's launch has ended. */
+int bd_gilmore_sweep_progress(const bd_gilmore_sweep* sw, long long out[5]);
+/* Host-mapped stop word of the handle: nonzero makes the running and later
+ * advances claim nothing new and suspend their cells; clear it to go on. */
+int* bd_gilmore_sweep_stop_word(bd_gilmore_sweep* sw);
+int bd_gilmore_sweep_destroy(bd_gilmore_sweep* sw);
 
 /* Queue order of bd_gilmore_integrate (the results do not depend on it):
  * INPUT hands bubbles out in input order.  SLICED runs passes of `slice`,

@@ -273,6 +273,7 @@ void orc_gilmore_reference_params(bd_gilmore_params* p) {  // G:440-460
     p->rel_eps = std::pow(10.0, -9);
     p->h0 = std::pow(10.0, -13);
     p->max_solve_steps = 1e7;
+    p->arithmetic = BD_GM_ARITH_REFERENCE;
 }
 
 void orc_dp_tableau(int tab, double* a, double* b, double* bb, double* c) {
@@ -401,6 +402,10 @@ void orc_gilmore_threshold(const bd_gilmore_params* g, double* results_A, const 
         m.d.f[0] = fr1; m.d.f[1] = fr2; m.d.f[2] = fr3;
         const long long cap = g->max_solve_steps > 0 ? (long long)g->max_solve_steps : 10000000LL;
         long long steps = 0;
+        // A solve that ends neither in collapse nor at endT within the cap
+        // leaves the search undecided: the cell's result is BD_GM_SWEEP_CAPPED
+        // (the reference has no cap and would keep integrating).
+        bool capped = false;
         // (1) 0.1 MPa steps; h carries over between amplitudes (G:374)
         while (!threshold_5 && Amplitude_5 < std::pow(10.0, 7)) {
             t = 0.0; R = R0; R1 = 0.0; Tau = 0.0; q = 0.0;
@@ -410,7 +415,9 @@ void orc_gilmore_threshold(const bd_gilmore_params* g, double* results_A, const 
             steps = 0;
             while (t < endT && !threshold_5 && steps++ < cap)
                 DPRK_step(m, t, R, R1, Tau, q, h, threshold_5, endT, g->abs_eps, g->rel_eps);
+            if (!threshold_5 && t < endT) { capped = true; break; }
         }
+        if (capped) { results_A[idx] = BD_GM_SWEEP_CAPPED; continue; }
         if (!threshold_5 && Amplitude_5 == std::pow(10.0, 7)) results_A[idx] = Amplitude_5;
         // (2) 0.01 MPa steps down
         Amplitude_4 = Amplitude_5;
@@ -423,8 +430,10 @@ void orc_gilmore_threshold(const bd_gilmore_params* g, double* results_A, const 
             steps = 0;
             while (t < endT && !threshold_4 && steps++ < cap)
                 DPRK_step(m, t, R, R1, Tau, q, h, threshold_4, endT, g->abs_eps, g->rel_eps);
+            if (!threshold_4 && t < endT) { capped = true; break; }
             if (!threshold_4) Amplitude_4 = Amplitude_4 + std::pow(10.0, 4);
         }
+        if (capped) { results_A[idx] = BD_GM_SWEEP_CAPPED; continue; }
         // (3) 0.001 MPa steps down
         Amplitude_3 = Amplitude_4;
         while (threshold_5 && !threshold_4 && threshold_3) {
@@ -436,8 +445,10 @@ void orc_gilmore_threshold(const bd_gilmore_params* g, double* results_A, const 
             steps = 0;
             while (t < endT && !threshold_3 && steps++ < cap)
                 DPRK_step(m, t, R, R1, Tau, q, h, threshold_3, endT, g->abs_eps, g->rel_eps);
+            if (!threshold_3 && t < endT) { capped = true; break; }
             if (!threshold_3) results_A[idx] = Amplitude_3 + std::pow(10.0, 3);
         }
+        if (capped) results_A[idx] = BD_GM_SWEEP_CAPPED;
     }
 }
 

@@ -16,22 +16,6 @@ def host(t):
     return t.cpu().numpy()
 
 
-def test_threshold_sweep_matches_oracle(orc):
-    # drop-in DPRK_solver_threshold_fix_tissue on 2 combos x 4 radii; the published
-    # tableau keeps the solves short (the reference tableau's arithmetic is pinned
-    # by the batch DP5(4) test below)
-    p = gm.reference_params()
-    p.tableau = 1
-    p.minR0, p.stepR0 = 1e-6, 1e-6
-    fr = np.array([[1e6, 1e6, 1e6], [1e6, 2e6, 3e6]])
-    n_r0 = 4
-    got = gm.threshold_sweep(p, fr, n_r0)
-    ref = orc.gilmore_threshold(p, fr, n_r0, 0, 2 * n_r0, nthreads=8)
-    print("GPU", got, "\nCPU", ref)
-    assert np.all(np.abs(got - ref) <= 1e3)
-    assert np.sum(got == ref) >= 6
-
-
 def test_threshold_sweep_range_prefix_and_stop():
     # the range sweep gives the drop-in's values on the whole table and on a
     # sub-range, with the exact completed count; a stop request set before the
@@ -39,8 +23,10 @@ def test_threshold_sweep_range_prefix_and_stop():
     import ctypes as C
     import time
     from bubbledynamics_amd import _ffi
+    # (the fast arithmetic: this test is about the range and stop mechanics)
     p = gm.reference_params()
     p.tableau = 1
+    p.arithmetic = gm.BD_GM_ARITH_FAST
     p.minR0, p.stepR0 = 1e-6, 1e-6
     fr = np.array([[1e6, 1e6, 1e6], [1e6, 2e6, 3e6], [2e6, 3e6, 5e6]])
     n_r0 = 4
@@ -108,6 +94,29 @@ def test_dp45_batch_matches_oracle(orc, tab):
     assert np.median(rel) <= 1e-9
     assert rel.max() <= 1e-6
     acc, acc_r = host(out["n_acc"]), ref["n_acc"]
+    assert np.all(np.abs(acc - acc_r) <= np.maximum(3, 0.01 * acc_r))
+    np.testing.assert_array_equal(host(out["t_collapse"]) > 0, ref["t_collapse"] > 0)
+
+
+@pytest.mark.parametrize("tab", [0, 1])
+def test_dp45_batch_reference_arithmetic_matches_oracle(orc, tab):
+    # the reference's expression order with the first stage recomputed: only
+    # libm rounding separates the GPU from the oracle
+    from bubbledynamics_amd._ffi import BD_GM_ARITH_REFERENCE
+    p = gm.workload_params(arithmetic=BD_GM_ARITH_REFERENCE)
+    p.tableau = tab
+    R0, amp, ph, fr = gm.config3_inputs(48, 3)
+    T = 3e-6
+    b = gm.BubbleBatch(R0, amp, ph, fr)
+    out = b.run(p, BD_GM_DP45, T, 1e-13, rtol=1e-8, atol=1e-12)
+    torch.cuda.synchronize()
+    ref = orc.gilmore_integrate(p, BD_GM_DP45, R0, amp, ph, fr, T, 1e-13, rtol=1e-8, atol=1e-12, nthreads=8)
+    rel = np.abs(host(out["Rmax"]) / ref["Rmax"] - 1)
+    acc, acc_r = host(out["n_acc"]), ref["n_acc"]
+    same = np.mean(acc == acc_r)
+    print(f"DP45 reference arithmetic: Rmax rel err max {rel.max():.3e} median {np.median(rel):.3e}; "
+          f"equal step counts {same:.3f}")
+    assert rel.max() <= 1e-8
     assert np.all(np.abs(acc - acc_r) <= np.maximum(3, 0.01 * acc_r))
     np.testing.assert_array_equal(host(out["t_collapse"]) > 0, ref["t_collapse"] > 0)
 

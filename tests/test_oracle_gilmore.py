@@ -200,3 +200,38 @@ def test_full_threshold_table_samples_match_oracle(orc):
     for idx, want in zip(fx["cells"], fx["gpu_threshold_pa"]):
         got = orc.gilmore_threshold(p, fr, n_r0, idx, idx + 1, nthreads=1)[idx]
         assert got == want, (idx, fr[idx // n_r0], got, want)
+
+
+def test_sweep_fixture_is_the_oracle(orc):
+    # tests/golden/sweep_cells.json (the GPU sweep's expected values) is what
+    # the oracle computes: the capped setting in full, two cells of the others
+    import json
+    from pathlib import Path
+    fix = json.loads((Path(__file__).parent / "golden" / "sweep_cells.json").read_text())
+    fr = np.array(fix["fr_combo"])
+    n_r0 = fix["n_r0"]
+    for name, cells in (("capped", range(len(fr) * n_r0)), ("dp1980", (4, 21)), ("reference", (9,))):
+        s = fix["settings"][name]
+        p = orc.gilmore_reference_params()
+        p.tableau, p.max_solve_steps = s["tableau"], s["max_solve_steps"]
+        p.abs_eps = p.rel_eps = s["eps"]
+        p.minR0, p.stepR0 = fix["min_r0"], fix["step_r0"]
+        for i in cells:
+            got = orc.gilmore_threshold(p, fr, n_r0, i, i + 1, nthreads=1)[i]
+            assert got == s["threshold_pa"][i], (name, i)
+    cap = fix["settings"]["capped"]["threshold_pa"]
+    assert -2.0 in cap and any(v > 0 for v in cap)
+
+
+def test_capped_solves_give_the_sentinel(orc):
+    # a search whose solve meets max_solve_steps undecided returns -2 (the
+    # reference has no cap); with a cap the solves never reach, the result
+    # is the ordinary threshold
+    p = orc.gilmore_reference_params()
+    p.tableau = 1
+    p.minR0, p.stepR0 = 1e-6, 1e-6
+    fr = np.array([[5e6, 5e6, 5e6]])
+    free = orc.gilmore_threshold(p, fr, 1, 0, 1, nthreads=1)[0]
+    assert free > 0
+    p.max_solve_steps = 10
+    assert orc.gilmore_threshold(p, fr, 1, 0, 1, nthreads=1)[0] == -2.0
