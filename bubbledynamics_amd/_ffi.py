@@ -167,6 +167,7 @@ SIGNATURES: dict[str, tuple] = {
     "bd_spiral_dwell": (_I, [_I, _D, _D, C.POINTER(_I), C.POINTER(_I)]),
     "bd_gilmore_reference_params": (None, [C.POINTER(GilmoreParams)]),
     "bd_gilmore_threshold_sweep": (_I, [C.POINTER(GilmoreParams), _VP, _VP, _I, _I, _I, _VP]),
+    "bd_gilmore_threshold_sweep_noalloc": (_I, [C.POINTER(GilmoreParams), _VP, _VP, _I, _I, _I, _VP]),
     "bd_gilmore_threshold_sweep_range": (_I, [C.POINTER(GilmoreParams), _VP, _VP, _I, _I, _I, _I, _VP]),
     "bd_gilmore_sweep_flags": (_IP, []),
     "bd_gilmore_sweep_create": (_I, [C.POINTER(GilmoreParams), _VP, _I, _I, _I, _I, C.POINTER(_VP)]),

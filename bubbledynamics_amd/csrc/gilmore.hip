@@ -31,6 +31,12 @@
 #include <type_traits>
 #include <vector>
 
+// Scratch of bd_gilmore_threshold_sweep_noalloc: the sweep's queue counters
+// (5 used), one copy per GPU for the life of the process.  In host code this
+// name is a host-side stand-in: kernels get the device address that
+// hipGetSymbolAddress returns for it (noalloc_scratch_ptr below), never the name.
+static __device__ __align__(256) int scratch[64];
+
 namespace bd {
 
 template <typename U>
@@ -810,6 +816,9 @@ static GmShape gm_shape(const bd_gilmore_batch* b) {
 
 // the drop-in sweep's counters, one set per device (its signature has no workspace)
 constexpr int kSweepCounters = 5;
+// cells of one sweep: claims are counted in 32-bit words, and every lane of
+// the persistent grid (2^19 at most) may claim once past the end
+constexpr long long kSweepMaxCells = 0x7fffffffLL - (1LL << 24);
 static int* sweep_counter() {
     static int* ctr[64] = {nullptr};
     std::lock_guard<std::mutex> lock(g_gm_mutex);
@@ -927,7 +936,7 @@ static int check_sweep(const bd_gilmore_params* prm, const double* results_A, co
     int rc = check_gilmore(prm);
     if (rc) return rc;
     if (!results_A || !Fr_combo || N_fr_combo <= 0 || N_R0 <= 0) return BD_ERR_INVALID;
-    if ((long long)N_fr_combo * N_R0 > 0x7fffffffLL) return BD_ERR_INVALID;
+    if ((long long)N_fr_combo * N_R0 > kSweepMaxCells) return BD_ERR_INVALID;
     return BD_OK;
 }
 
@@ -941,6 +950,45 @@ extern "C" int bd_gilmore_threshold_sweep(const bd_gilmore_params* prm, double* 
     if (!ctr) return BD_ERR_ALLOC;
     GmSweepArgs a;
     sweep_args(a, prm, results_A, Fr_combo, N_fr_combo, N_R0, ctr, 0, N_fr_combo * N_R0);
+    return launch_sweep(a, prm->arithmetic, as_stream(stream));
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept; nullptr if the lookup fails.
+static int* noalloc_scratch_ptr() {
+    static int* ptr[kGmMaxDevices] = {nullptr};
+    std::lock_guard<std::mutex> lock(g_gm_mutex);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kGmMaxDevices) return nullptr;
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            fprintf(stderr, "bubbledynamics: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+            return nullptr;
+        }
+        ptr[dev] = (int*)p;
+    }
+    return ptr[dev];
+}
+
+extern "C" int bd_gilmore_threshold_sweep_noalloc(const bd_gilmore_params* prm, double* results_A,
+                                                  const double* Fr_combo, int N_fr_combo, int N_R0, int N_A,
+                                                  void* stream) {
+    (void)N_A;
+    int rc = check_gilmore(prm);
+    if (rc) return rc;
+    if (!results_A || !Fr_combo || N_fr_combo <= 0 || N_R0 <= 0) return BD_ERR_INVALID;
+    if ((long long)N_fr_combo * N_R0 > kSweepMaxCells) {
+        fprintf(stderr, "bubbledynamics: bd_gilmore_threshold_sweep_noalloc: %lld cells exceed the maximum %lld "
+                        "(32-bit queue counters)\n", (long long)N_fr_combo * N_R0, (long long)kSweepMaxCells);
+        return BD_ERR_INVALID;
+    }
+    int* ctr = noalloc_scratch_ptr();
+    if (!ctr) return BD_ERR_HIP;
+    GmSweepArgs a;
+    sweep_args(a, prm, results_A, Fr_combo, N_fr_combo, N_R0, ctr, 0, N_fr_combo * N_R0);
+    // launch_sweep clears the counters with a fill kernel before the sweep kernel
     return launch_sweep(a, prm->arithmetic, as_stream(stream));
 }
 
@@ -998,7 +1046,7 @@ extern "C" int bd_gilmore_sweep_create(const bd_gilmore_params* prm, const doubl
     *out = nullptr;
     int rc = check_gilmore(prm);
     if (rc) return rc;
-    if (!Fr_combo || N_fr_combo <= 0 || N_R0 <= 0 || (long long)N_fr_combo * N_R0 > 0x7fffffffLL)
+    if (!Fr_combo || N_fr_combo <= 0 || N_R0 <= 0 || (long long)N_fr_combo * N_R0 > kSweepMaxCells)
         return BD_ERR_INVALID;
     if (cell0 < 0 || cell_end > N_fr_combo * N_R0 || cell0 >= cell_end) return BD_ERR_INVALID;
     auto* sw = new bd_gilmore_sweep();

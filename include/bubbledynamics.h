@@ -331,7 +331,8 @@ void bd_gilmore_reference_params(bd_gilmore_params* p);
  * cavitation threshold amplitude found in 0.1 / 0.01 / 0.001 MPa stages, -1
  * if none is set, 1e7 if no collapse up to 10 MPa, BD_GM_SWEEP_CAPPED if a
  * solve reached max_solve_steps.  Fr_combo: 3*N_fr_combo frequencies
- * (device).  N_A is accepted and ignored, as in the reference.  The queue
+ * (device); at most 2^31 - 2^24 - 1 cells (32-bit queue counters).  N_A is
+ * accepted and ignored, as in the reference.  The queue
  * counter is one per device: sweeps that overlap in time on one device (other
  * streams or host threads) race; run them one after another. */
 int bd_gilmore_threshold_sweep(const bd_gilmore_params* prm, double* results_A,
@@ -370,6 +371,20 @@ typedef struct bd_gilmore_batch {
     int32_t slice;            /* BD_GM_SCHED_SLICED: attempts of the first pass (0 = 128) */
     double reserved[3];
 } bd_gilmore_batch;
+
+/* Allocation-free variant of bd_gilmore_threshold_sweep (no counterpart in
+ * the reference): same arguments, results and kernels, but the sweep's queue
+ * counters live in a static device array of the library (256 bytes, one per
+ * GPU, kept until the process ends) instead of memory the library allocates
+ * on a device's first call, so it never calls hipMalloc -- for callers that
+ * cannot allocate (stream capture into a graph, serving loops that forbid it).
+ * At most 2^31 - 2^24 - 1 cells (N_fr_combo * N_R0); more return
+ * BD_ERR_INVALID with a message on stderr.  Every call on a GPU shares that
+ * array: calls that overlap in time on one GPU race; run them one after
+ * another. */
+int bd_gilmore_threshold_sweep_noalloc(const bd_gilmore_params* prm, double* results_A,
+                                       const double* Fr_combo, int N_fr_combo, int N_R0, int N_A,
+                                       void* stream);
 
 /* The same sweep over the cells idx in [cell0, cell_end) only, for runs too
  * long for one sitting (no counterpart in the reference; the full 27,775 x 100
