@@ -4,8 +4,12 @@
 A step is one Westervelt time step over the 192^3 grid (interior 6th-order
 update with relaxation and nonlinearity, source plane, Mur faces, sponge,
 peak pressure, per-step energy and the focus sample) -- one kernel launch,
-replayed from a pair of captured 6-step hipGraphs.  Inputs live in HBM
-before timing starts.
+replayed from a pair of 6-step hipGraphs captured when the plan is built.
+Inputs live in HBM before timing starts.  After --warmup steps, --reps (>= 5)
+runs of exactly --steps steps are timed with HIP events and the median is
+reported.  The FP32-vs-FP64 accuracy comes from the whole 3,000-step tone
+burst in both precisions, whatever --steps is.  The CPU baseline is the FP64
+oracle on the host (the per-GPU core share and one core), with lscpu.
 
 Prints one JSON line.  `--gpus N` with N > 1 exits: one GPU is measured (the
 z-slab decomposition across GPUs is not built).
@@ -46,33 +50,103 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_description() -> dict:
+    """CPU of the host: visible CPUs, the threads this process may use and lscpu."""
+    import subprocess
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)",
+                             "CPU max MHz", "L3 cache"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_threads() -> int:
+    """Threads of the CPU baseline: the host CPUs this process may use, at most
+    the per-GPU share the box gives one GPU's jobs (OMP_NUM_THREADS, 16)."""
+    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return max(1, min(len(os.sched_getaffinity(0)), share))
+
+
 def cpu_baseline(n: int, budget_s: float) -> dict:
-    """FP64 oracle (serial C++ restatement) on the host cores, bounded sample."""
+    """FP64 oracle (serial C++ restatement, OpenMP over z-planes) on the host
+    cores, on a bounded sample of the same 192^3 workload: all threads of the
+    share, then one thread."""
     import numpy as np
 
     import oracle
     from bubbledynamics_amd import presets
     orc = oracle.load()
     p = presets.tone_burst_192(n)
-    cores = min(len(os.sched_getaffinity(0)), 16)
     state = np.zeros((12, n, n, n))
     pmax = np.zeros((n, n, n))
-    t0 = time.perf_counter()
-    steps = 0
-    # warm-up step (page faults), not timed
-    orc.wv_run(p, n, n, 0, 1, state, pmax, None, None, (-1, -1, -1), None, None, cores)
-    steps = 1
-    t0 = time.perf_counter()
-    done = 0
-    while time.perf_counter() - t0 < budget_s or done < 3:
-        orc.wv_run(p, n, n, steps, 1, state, pmax, None, None, (-1, -1, -1), None, None, cores)
+
+    def rate(threads, budget, step0):
+        steps = step0
+        orc.wv_run(p, n, n, steps, 1, state, pmax, None, None, (-1, -1, -1), None, None, threads)  # warm-up
         steps += 1
-        done += 1
-    el = time.perf_counter() - t0
-    val = n ** 3 * done / el / 1e6
-    return {"value": round(val, 3), "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
-            "sample": f"FP64 oracle, same 192^3 workload, {done} steps after 1 warm-up step, "
-                      f"{el:.1f} s wall, OpenMP over z-planes"}
+        t0 = time.perf_counter()
+        done = 0
+        while time.perf_counter() - t0 < budget or done < 2:
+            orc.wv_run(p, n, n, steps, 1, state, pmax, None, None, (-1, -1, -1), None, None, threads)
+            steps += 1
+            done += 1
+        el = time.perf_counter() - t0
+        return n ** 3 * done / el / 1e6, done, el, steps
+
+    cores = cpu_threads()
+    v_all, d_all, e_all, s = rate(cores, budget_s, 0)
+    v_one, d_one, e_one, _ = rate(1, budget_s / 2, s)
+    return {"value": round(v_all, 3), "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
+            "serial_1core": round(v_one, 3),
+            "sample": f"FP64 oracle, same 192^3 workload: {d_all} steps on {cores} threads in {e_all:.1f} s "
+                      f"and {d_one} steps on 1 thread in {e_one:.1f} s, each after 1 warm-up step",
+            "host": host_description()}
+
+
+def gilmore_cpu_baseline(budget_s: float) -> dict:
+    """FP64 oracle timed on the host: config 0 (RK4 dt = 0.1 ns, 20 us) on its
+    first bubbles and config 3 (DP5(4), 20 us) on a 4,096-bubble subset
+    stratified in R0; rates in bubble-steps/s."""
+    import oracle
+    from bubbledynamics_amd import gilmore as gm
+    orc = oracle.load()
+    p = gm.workload_params()
+    cores = cpu_threads()
+    R0, amp, ph, fr = gm.config0_inputs(1024, 0)
+    k = max(cores, 16)
+    t0 = time.perf_counter()
+    r0 = orc.gilmore_integrate(p, gm.BD_GM_RK4, R0[:k], amp[:k], ph[:k], fr, 20e-6, 1e-10, nthreads=cores)
+    e0 = time.perf_counter() - t0
+    idx = config3_subset()
+    R0, amp, ph, fr = gm.config3_inputs(1_000_000, 3)
+    t0 = time.perf_counter()
+    r3 = orc.gilmore_integrate(p, gm.BD_GM_DP45, R0[idx], amp[idx], ph[idx], fr, 20e-6, 1e-13, rtol=1e-8,
+                               atol=1e-12, nthreads=cores)
+    e3 = time.perf_counter() - t0
+    s3 = int(r3["n_acc"].sum())
+    return {"unit": "bubble-steps/s", "cores": cores, "kind": "port",
+            "rk4_1024": {"value": float(r0["n_acc"].sum()) / e0,
+                         "sample": f"first {k} of the 1,024 config-0 bubbles, full 20 us, {e0:.1f} s"},
+            "dp45_1e6": {"value": s3 / e3, "accepted_steps": s3,
+                         "sample": f"4,096 config-3 bubbles stratified in R0, full 20 us, {e3:.1f} s"},
+            "dp45_subset_rmax": r3["Rmax"].tolist(), "dp45_subset_collapsed": (r3["t_collapse"] > 0).tolist()}
+
+
+def config3_subset(n: int = 1_000_000, k: int = 4096):
+    """Indices of k of the n config-3 bubbles, stratified in R0: the bubbles at
+    ranks (j + 1/2) n/k of the R0 order."""
+    import numpy as np
+    from bubbledynamics_amd import gilmore as gm
+    R0, _, _, _ = gm.config3_inputs(n, 3)
+    order = np.argsort(R0, kind="stable")
+    return np.sort(order[((np.arange(k) + 0.5) * n / k).astype(np.int64)])
 
 
 def read_traffic(profile: Path) -> float | None:
@@ -98,9 +172,11 @@ def _timed(fn):
     return out, e0.elapsed_time(e1) / 1e3
 
 
-def gilmore_figures() -> dict:
+def gilmore_figures(cpu_budget_s: float | None) -> dict:
     """1e6 bubbles, adaptive DP5(4) (rtol 1e-8, atol 1e-12), 20 us, FP64 and the
-    FP32/FP64 split; 1,024 bubbles, RK4 dt = 0.1 ns, 20 us, FP64 with R(t) every 1 ns."""
+    FP32/FP64 split; 1,024 bubbles, RK4 dt = 0.1 ns, 20 us, FP64 with R(t) every
+    1 ns; the FP64 oracle on the host for both (cpu_budget_s None: skipped),
+    with the GPU's Rmax on the stratified 4,096-bubble subset against it."""
     import numpy as np
     from bubbledynamics_amd import gilmore as gm
     from bubbledynamics_amd._ffi import BD_GM_DP45, BD_GM_RK4, BD_GM_SCHED_SLICED, BD_GM_SPLIT
@@ -125,11 +201,11 @@ def gilmore_figures() -> dict:
     bc = gm.BubbleBatch(R0c, ampc, phc, frc)
     oc, tc = _timed(lambda: bc.run(p, BD_GM_RK4, 20e-6, 1e-10, out_every=10, n_out=20001))
     sc = int(oc["n_acc"].sum().item())
-    return {
+    out = {
         "dp45_1e6_fp64": {"bubble_steps_per_s": s64 / t64, "accepted_steps": s64, "rejected": rej64,
                           "seconds": t64, "collapsed_frac": float((o64["t_collapse"] > 0).float().mean().item()),
                           "autotuned_shape": tune, "autotune_seconds": t_tune, "schedule": "sliced (128, 2048, rest)",
-                          "input_order_bubble_steps_per_s": s64 / t64_input},
+                          "input_order_bubble_steps_per_s": s64 / t64_input, "arithmetic": "fast"},
         "dp45_1e6_split": {"bubble_steps_per_s": s32 / t32, "accepted_steps": s32, "seconds": t32,
                            "rmax_rel_err_vs_fp64": {"median": float(np.median(rel)),
                                                     "p90": float(np.quantile(rel, 0.9)),
@@ -139,6 +215,21 @@ def gilmore_figures() -> dict:
                           "collapsed_frac": float((oc["t_collapse"] > 0).float().mean().item())},
         "unit": "bubble-steps/s", "tableau": "Dormand-Prince 1980",
     }
+    if cpu_budget_s is not None:
+        cpu = gilmore_cpu_baseline(cpu_budget_s)
+        idx = config3_subset()
+        ref = np.array(cpu.pop("dp45_subset_rmax"))
+        ref_c = np.array(cpu.pop("dp45_subset_collapsed"))
+        got = o64["Rmax"].cpu().numpy()[idx]
+        got_c = (o64["t_collapse"] > 0).cpu().numpy()[idx]
+        e = np.abs(got / ref - 1)
+        out["dp45_1e6_fp64"]["rmax_rel_err_vs_oracle_subset4096"] = {
+            "median": float(np.median(e)), "p99": float(np.quantile(e, 0.99)), "max": float(e.max()),
+            "collapse_flag_flips": int(np.sum(got_c != ref_c))}
+        cpu["rk4_1024"]["gpu_x"] = (sc / tc) / cpu["rk4_1024"]["value"]
+        cpu["dp45_1e6"]["gpu_x"] = (s64 / t64) / cpu["dp45_1e6"]["value"]
+        out["cpu_baseline"] = cpu
+    return out
 
 
 def adjoint_figures() -> dict:
@@ -405,7 +496,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=60)
     ap.add_argument("--n", type=int, default=192)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--reps", type=int, default=5, help="timed runs of --steps steps (median reported; >= 5)")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
@@ -472,36 +564,40 @@ def main() -> int:
         torch.cuda.synchronize()
         del src, dst
     n, K, W = args.n, args.steps, args.warmup
+    reps = max(5, args.reps)
     p = presets.tone_burst_192(n)
     focus = presets.tone_burst_focus(p, n)
-    total = W + K
+    total = W + reps * K
 
-    def one_run(dtype):
+    def timed_runs(dtype):
+        # the plan captures both 6-step graphs when it is built, so the timed
+        # region holds kernel launches and graph replays only
         run = WesterveltRun(p, n, n, dtype=dtype, max_steps=total + 1, focus=focus)
         run.run(W)
         torch.cuda.synchronize()
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        ev0.record()
-        run.run(K)
-        ev1.record()
-        torch.cuda.synchronize()
-        ms = ev0.elapsed_time(ev1)
-        return run, ms
+        times = []
+        for _ in range(reps):
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            ev0.record()
+            run.run(K)
+            ev1.record()
+            torch.cuda.synchronize()
+            times.append(ev0.elapsed_time(ev1))
+        return run, times
 
-    log(f"FP32 run {n}^3: {W} warm-up + {K} timed steps")
-    run32, ms = one_run(torch.float32)
+    log(f"FP32 run {n}^3: {W} warm-up steps, then {reps} timed runs of {K} steps")
+    run32, times = timed_runs(torch.float32)
+    ms = float(np.median(times))
     value = n ** 3 * K / (ms / 1e3) / 1e6
-    log(f"FP32 {value:.1f} Mcell-steps/s ({ms / K * 1e3:.2f} us/step)")
-
-    run32.finalize()
-    torch.cuda.synchronize()
-    pmax32 = run32.pmax.clone()
+    log(f"FP32 {value:.1f} Mcell-steps/s ({ms / K * 1e3:.2f} us/step; runs {[round(t, 3) for t in times]} ms)")
     shape = run32.shape
-    # The timed region launches exactly one kernel per step (the graphs hold
-    # nothing else), so its HIP-event time per step bounds that kernel's
-    # average duration from above; the roofline uses it.  For comparison, the
-    # same kernel relaunched alone on the stream it runs on (these launches
-    # recompute the next step and touch pmax, hence the copy above).
+    # The timed region launches one step kernel per step (eager until the ring
+    # phase is 0 mod 6, then graph replays) and, at the end of each run, the
+    # fold of the levels after the last 6-step pmax window; its time per step
+    # bounds the step kernel's average duration from above and the roofline
+    # uses it.  For comparison, the same kernel relaunched alone on the stream
+    # it runs on (these launches recompute the next step and touch pmax).
     step_ms = ms / K
     R = 50
     lib = _ffi.load()
@@ -522,35 +618,60 @@ def main() -> int:
     # cell: read p^n, write p^{n+1}, pmax read/modify/write (FP32)
     alg_bytes = cells * 14 * 4 + edge_cells * 4 * 4
     achieved = alg_bytes / (step_ms / 1e3)
-    log(f"step kernel {step_ms * 1e3:.2f} us in the timed region ({k_ms * 1e3:.2f} us relaunched alone) "
+    log(f"step kernel {step_ms * 1e3:.2f} us per step in the timed region ({k_ms * 1e3:.2f} us relaunched alone) "
         f"-> {achieved / 1e9:.0f} GB/s algorithmic; shape {shape}")
     kname = "wv_march<float,%d,%d,%d,%s,%s>" % (
         shape["tile_x"], shape["tile_y"], shape["rows"], "true" if shape["prefetch"] else "false",
         "true" if shape["nontemporal"] else "false")
+    del run32
+    torch.cuda.empty_cache()
 
-    f32 = run32.focus.double().cpu().numpy()[:total]
-    energy = run32.energy.cpu().numpy()[:total]
-    harm = focal_harmonics(f32, p.dt, p.fr, 3)
-
-    extra = {}
-    if not args.no_fp64:
-        log("FP64 run (same workload)")
-        run64, ms64 = one_run(torch.float64)
-        run64.finalize()
+    # Accuracy leg, independent of --steps: the whole 3,000-step tone burst
+    # (the burst reaches the focus after ~500 steps) in FP32 and in FP64 from
+    # rest; focal record, pmax, energy and focal harmonics.
+    acc_steps = 3000
+    full = {}
+    for name, dt_ in (("f32", torch.float32), ("f64", torch.float64)):
+        if name == "f64" and args.no_fp64:
+            continue
+        r = WesterveltRun(p, n, n, dtype=dt_, max_steps=acc_steps, focus=focus)
         torch.cuda.synchronize()
-        f64 = run64.focus.cpu().numpy()[:total]
-        rel = float(np.linalg.norm(f32 - f64) / np.linalg.norm(f64))
-        extra["fp64"] = {"value": round(n ** 3 * K / (ms64 / 1e3) / 1e6, 1),
-                         "unit": "Mcell-steps/s",
-                         "focal_rel_l2_fp32_vs_fp64": rel,
-                         "pmax_rel_l2_fp32_vs_fp64": float(
-                             (torch.linalg.vector_norm(pmax32.double() - run64.pmax) /
-                              torch.linalg.vector_norm(run64.pmax)).item())}
-        log(f"FP64 {extra['fp64']['value']} Mcell-steps/s; focal rel L2 FP32 vs FP64 = {rel:.2e}")
-        del run64
+        t0 = time.perf_counter()
+        r.run(acc_steps)
+        r.finalize()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        full[name] = {"seconds": el, "focus": r.focus.double().cpu().numpy(), "pmax": r.pmax.double(),
+                      "energy": r.energy.cpu().numpy()}
+        del r
+        log(f"{name} whole {acc_steps}-step run: {el:.3f} s ({n ** 3 * acc_steps / el / 1e6:.0f} Mcell-steps/s)")
+    f32 = full["f32"]["focus"]
+    harm = focal_harmonics(f32, p.dt, p.fr, 3)
+    extra = {"run_3000": {"steps": acc_steps, "f32_seconds": round(full["f32"]["seconds"], 4),
+                          "f32_mcell_steps_per_s": round(n ** 3 * acc_steps / full["f32"]["seconds"] / 1e6, 1)}}
+    accuracy = {"focal_rel_l2_fp32_vs_fp64": None, "pmax_rel_l2_fp32_vs_fp64": None,
+                "record": f"focus sample of every step of the whole {acc_steps}-step run"}
+    if "f64" in full:
+        f64 = full["f64"]["focus"]
+        nf = float(np.linalg.norm(f64))
+        accuracy["focal_rel_l2_fp32_vs_fp64"] = float(np.linalg.norm(f32 - f64) / nf) if nf > 0 else None
+        p64 = full["f64"]["pmax"]
+        accuracy["pmax_rel_l2_fp32_vs_fp64"] = float(
+            (torch.linalg.vector_norm(full["f32"]["pmax"] - p64) / torch.linalg.vector_norm(p64)).item())
+        extra["run_3000"]["f64_seconds"] = round(full["f64"]["seconds"], 4)
+        extra["run_3000"]["f64_mcell_steps_per_s"] = round(n ** 3 * acc_steps / full["f64"]["seconds"] / 1e6, 1)
+        extra["run_3000"]["fp32_speedup_vs_fp64"] = round(full["f64"]["seconds"] / full["f32"]["seconds"], 3)
+        log(f"FP32 vs FP64 over {acc_steps} steps: focal rel L2 {accuracy['focal_rel_l2_fp32_vs_fp64']}, "
+            f"pmax rel L2 {accuracy['pmax_rel_l2_fp32_vs_fp64']:.2e}")
+    energy = full["f32"]["energy"]
+    focus_out = {"peak_pa": float(np.abs(f32).max()), "harmonics_pa": [float(h) for h in harm],
+                 "energy_max_j": float(energy[:, 0].max())}
+    del full
+    torch.cuda.empty_cache()
 
     if not args.no_extras:
-        for name, fn in (("gilmore", gilmore_figures), ("adjoint", adjoint_figures),
+        gil_cpu = None if args.no_cpu else args.cpu_seconds
+        for name, fn in (("gilmore", lambda: gilmore_figures(gil_cpu)), ("adjoint", adjoint_figures),
                          ("coupled_bioheat", coupled_figures)):
             try:
                 log(f"{name} ...")
@@ -562,9 +683,11 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        log(f"CPU baseline (~{args.cpu_seconds:.0f} s)")
+        log(f"CPU baseline (~{1.5 * args.cpu_seconds:.0f} s)")
         try:
             cpu = cpu_baseline(n, args.cpu_seconds)
+            cpu["gpu_x"] = round(value / cpu["value"], 1)
+            cpu["gpu_x_serial"] = round(value / cpu["serial_1core"], 1)
         except Exception as exc:  # the baseline is reported, never required
             log(f"cpu baseline failed: {exc}")
 
@@ -576,26 +699,28 @@ def main() -> int:
         "n_gpus": 1,
         "steps": K,
         "warmup": W,
-        "ms_per_step": ms / K,
+        "ms_per_step": step_ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
+        "timing": {"repetitions": reps, "statistic": "median", "run_ms": [round(t, 4) for t in times]},
         "config": {"workload": f"westervelt_tone_burst_{n}", "grid": [n, n, n], "dx_m": p.dxy,
                    "dt_s": p.dt, "relaxation": 2, "source": "cap F#1, 1 MHz, 8 cycles, 1 MPa",
-                   "boundary": "Mur + 20-cell sponge", "hipgraph": "2 x 6-step"},
+                   "boundary": "Mur + 20-cell sponge", "hipgraph": "2 x 6-step, captured before timing"},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4),
                      "traffic": traffic,
                      "kernel": kname, "kernel_us": round(step_ms * 1e3, 2),
                      "kernel_us_relaunched_alone": round(k_ms * 1e3, 2),
+                     "frac_relaunched_alone": round(alg_bytes / (k_ms / 1e3) / PEAK_HBM, 4),
                      "workgroups": shape["workgroups"],
                      "alg_bytes_per_launch": alg_bytes, "cells_per_launch": n ** 3,
                      "inside_cells_per_launch": cells},
         "cpu_baseline": cpu,
-        "focus": {"peak_pa": float(np.abs(f32).max()), "harmonics_pa": [float(h) for h in harm],
-                  "energy_max_j": float(energy[:, 0].max())},
+        "accuracy": accuracy,
+        "focus": focus_out,
     }
     out.update(extra)
     print(json.dumps(out), flush=True)

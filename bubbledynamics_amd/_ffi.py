@@ -144,6 +144,7 @@ SIGNATURES: dict[str, tuple] = {
     "bd_westervelt_plan_create": (_I, [_PP, _I, _I, _I, _I, _I, _I, _I, C.POINTER(_VP)]),
     "bd_westervelt_plan_bind": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "bd_westervelt_plan_run": (_I, [_VP, _I, _I, _VP]),
+    "bd_westervelt_plan_prepare": (_I, [_VP]),
     "bd_westervelt_plan_finalize": (_I, [_VP, _VP]),
     "bd_westervelt_plan_steps": (_I, [_VP]),
     "bd_westervelt_plan_seek": (_I, [_VP, _I]),

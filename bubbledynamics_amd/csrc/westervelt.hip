@@ -319,6 +319,9 @@ static bool streams_past_mall(int Nxy, int Nz, size_t elem_bytes) {
 static int check_params(const bd_westervelt_params* p, int dtype, int Nxy, int Nz) {
     if (!p || (dtype != BD_F32 && dtype != BD_F64)) return BD_ERR_INVALID;
     if (Nxy < 10 || Nz < 10) return BD_ERR_INVALID;
+    // the step kernel addresses a plane and its z-neighbours with 32-bit byte
+    // offsets from a per-plane buffer descriptor (up to 4 planes past it)
+    if ((long long)Nxy * Nxy * 4 * (dtype == BD_F64 ? 8 : 4) > 0xFFFFFFFFLL) return BD_ERR_INVALID;
     if (!(p->dt > 0) || !(p->dxy > 0) || !(p->dz > 0)) return BD_ERR_INVALID;
     if (p->boundary != BD_BC_MUR && p->boundary != BD_BC_ZERO) return BD_ERR_INVALID;
     if (p->source != BD_SRC_CAP && p->source != BD_SRC_ARRAY) return BD_ERR_INVALID;
@@ -793,6 +796,34 @@ __global__ void pmax_fold_kernel(T* __restrict__ pmax, const T* __restrict__ S, 
     }
 }
 
+// Captures and instantiates the pair of 6-step graphs if they do not exist
+// yet.  Graph k reads step counter k and its last kernel writes counter 1-k
+// (= counter k + 6) for the next replay, so no kernel ever writes the counter
+// it reads.  Capture runs on a stream of its own; nothing is launched.
+template <typename T>
+static int plan_capture_t(bd_westervelt_plan* pl) {
+    for (int k = 0; k < 2; ++k) {
+        if (pl->gexec[k]) continue;
+        hipStream_t cap;
+        BD_HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        BD_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        int rc = BD_OK;
+        // replays start at a step n = 0 mod 6 inside the run, so the last
+        // kernel's window (levels n+1 .. n+6) lies inside the run
+        for (int j = 0; j < 6 && rc == BD_OK; ++j)
+            rc = plan_step<T>(pl, j, pl->d_step + k, j, cap, j == 5 ? 63 : 0,
+                              j == 5 ? pl->d_step + (1 - k) : nullptr);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(cap, &g);
+        (void)hipStreamDestroy(cap);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return BD_ERR_HIP;
+        pl->graph[k] = g;
+        BD_HIP_CHECK(hipGraphInstantiate(&pl->gexec[k], g, nullptr, nullptr, 0));
+    }
+    return BD_OK;
+}
+
 template <typename T>
 static int plan_run_t(bd_westervelt_plan* pl, int n_steps, int use_graph, hipStream_t st) {
     int n = pl->steps;
@@ -806,28 +837,8 @@ static int plan_run_t(bd_westervelt_plan* pl, int n_steps, int use_graph, hipStr
         if (!use_graph) continue;
     }
     if (use_graph && end - n >= 6) {
-        // A pair of 6-step graphs: graph k reads step counter k and its last
-        // kernel writes counter 1-k (= counter k + 6) for the next replay, so
-        // no kernel ever writes the counter it reads.
-        for (int k = 0; k < 2; ++k) {
-            if (pl->gexec[k]) continue;
-            hipStream_t cap;
-            BD_HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-            BD_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-            int rc = BD_OK;
-            // replays start at a step n = 0 mod 6 inside the run, so the last
-            // kernel's window (levels n+1 .. n+6) lies inside the run
-            for (int j = 0; j < 6 && rc == BD_OK; ++j)
-                rc = plan_step<T>(pl, j, pl->d_step + k, j, cap, j == 5 ? 63 : 0,
-                                  j == 5 ? pl->d_step + (1 - k) : nullptr);
-            hipGraph_t g = nullptr;
-            hipError_t e = hipStreamEndCapture(cap, &g);
-            (void)hipStreamDestroy(cap);
-            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-            if (e != hipSuccess) return BD_ERR_HIP;
-            pl->graph[k] = g;
-            BD_HIP_CHECK(hipGraphInstantiate(&pl->gexec[k], g, nullptr, nullptr, 0));
-        }
+        int rc = plan_capture_t<T>(pl);
+        if (rc) return rc;
         hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, st, pl->d_step, n);
         BD_LAUNCH_CHECK();
         for (int k = 0; end - n >= 6; k ^= 1) {
@@ -865,6 +876,15 @@ extern "C" int bd_westervelt_plan_run(bd_westervelt_plan* pl, int n_steps, int u
     if (dev != pl->device) return BD_ERR_INVALID;
     if (pl->dtype == BD_F32) return plan_run_t<float>(pl, n_steps, use_graph, as_stream(stream));
     return plan_run_t<double>(pl, n_steps, use_graph, as_stream(stream));
+}
+
+extern "C" int bd_westervelt_plan_prepare(bd_westervelt_plan* pl) {
+    if (!pl || !pl->state) return BD_ERR_INVALID;
+    int dev = 0;
+    BD_HIP_CHECK(hipGetDevice(&dev));
+    if (dev != pl->device) return BD_ERR_INVALID;
+    if (pl->dtype == BD_F32) return plan_capture_t<float>(pl);
+    return plan_capture_t<double>(pl);
 }
 
 template <typename T>

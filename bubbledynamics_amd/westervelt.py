@@ -43,7 +43,7 @@ class WesterveltRun:
     def __init__(self, params: WesterveltParams, nxy: int, nz: int, *, dtype=torch.float32,
                  max_steps: int = 1, focus: tuple[int, int, int] | None = None,
                  pmax: bool = True, energy: bool = True, elem_amp=None, elem_phase=None,
-                 device: str | torch.device = "cuda"):
+                 device: str | torch.device = "cuda", prepare_graphs: bool = True):
         _ffi.load()
         self.params = params.copy()
         self.nxy, self.nz, self.dtype = nxy, nz, dtype
@@ -66,6 +66,8 @@ class WesterveltRun:
         self._plan = plan
         call("bd_westervelt_plan_bind", plan, _ptr(self.state), _ptr(self.pmax),
              _ptr(self.energy), _ptr(self.focus), _ptr(self.elem_amp), _ptr(self.elem_phase))
+        if prepare_graphs:
+            call("bd_westervelt_plan_prepare", plan)
 
     # -- driving -----------------------------------------------------------
     def run(self, n_steps: int, use_graph: bool = True) -> None:

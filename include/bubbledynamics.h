@@ -162,6 +162,11 @@ int bd_westervelt_plan_bind(bd_westervelt_plan* plan, void* state, void* pmax, d
  * the remaining levels); levels present before a call (e.g. restored after
  * bd_westervelt_plan_seek) are not taken into pmax. */
 int bd_westervelt_plan_run(bd_westervelt_plan* plan, int n_steps, int use_graph, void* stream);
+/* Captures and instantiates the plan's pair of 6-step hipGraphs now (after
+ * bind; a later bind drops them), so that no later bd_westervelt_plan_run
+ * with use_graph=1 spends host time on graph construction.  Optional: run
+ * builds them on first use otherwise.  Launches nothing; synchronous. */
+int bd_westervelt_plan_prepare(bd_westervelt_plan* plan);
 /* Reduces the per-block energy partials of all steps run so far into the
  * bound energy buffer. */
 int bd_westervelt_plan_finalize(bd_westervelt_plan* plan, void* stream);

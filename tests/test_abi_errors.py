@@ -43,6 +43,20 @@ def test_westervelt_parameter_checks(lib):
     assert lib.bd_acoustic_power_water(C.byref(p), BD_F64, NULL, NULL, NULL, 40, 48, 0, NULL) == BD_ERR_INVALID
 
 
+def test_westervelt_plane_size_limit(lib):
+    # the step kernel forms 32-bit byte offsets over 4 planes from a plane's
+    # descriptor: 4 * Nxy^2 * sizeof(T) must stay below 4 GiB
+    p = presets.reference_scaled(40, 48)
+    plan = C.c_void_p()
+    for dtype, nxy in ((BD_F32, 16384), (BD_F64, 11586)):
+        assert 4 * nxy * nxy * (4 if dtype == BD_F32 else 8) > 0xFFFFFFFF
+        q = p.copy()
+        q.dt = p.dt * 40 / nxy       # keep the step stable at the finer spacing
+        q.dxy = p.dxy * 40 / nxy
+        assert lib.bd_westervelt_plan_create(C.byref(q), dtype, nxy, 48, 10, -1, -1, -1, C.byref(plan)) \
+            == BD_ERR_INVALID
+
+
 def test_harmonic_bin_checks(lib):
     h = (C.c_double * 4)()
     args = dict(dtype=BD_F64, H=C.cast(h, C.c_void_p), P=C.cast(h, C.c_void_p))
