@@ -124,7 +124,7 @@ def gilmore_cpu_baseline(budget_s: float) -> dict:
     t0 = time.perf_counter()
     r0 = orc.gilmore_integrate(p, gm.BD_GM_RK4, R0[:k], amp[:k], ph[:k], fr, 20e-6, 1e-10, nthreads=cores)
     e0 = time.perf_counter() - t0
-    idx = config3_subset()
+    idx = gm.config3_subset()
     R0, amp, ph, fr = gm.config3_inputs(1_000_000, 3)
     t0 = time.perf_counter()
     r3 = orc.gilmore_integrate(p, gm.BD_GM_DP45, R0[idx], amp[idx], ph[idx], fr, 20e-6, 1e-13, rtol=1e-8,
@@ -137,16 +137,6 @@ def gilmore_cpu_baseline(budget_s: float) -> dict:
             "dp45_1e6": {"value": s3 / e3, "accepted_steps": s3,
                          "sample": f"4,096 config-3 bubbles stratified in R0, full 20 us, {e3:.1f} s"},
             "dp45_subset_rmax": r3["Rmax"].tolist(), "dp45_subset_collapsed": (r3["t_collapse"] > 0).tolist()}
-
-
-def config3_subset(n: int = 1_000_000, k: int = 4096):
-    """Indices of k of the n config-3 bubbles, stratified in R0: the bubbles at
-    ranks (j + 1/2) n/k of the R0 order."""
-    import numpy as np
-    from bubbledynamics_amd import gilmore as gm
-    R0, _, _, _ = gm.config3_inputs(n, 3)
-    order = np.argsort(R0, kind="stable")
-    return np.sort(order[((np.arange(k) + 0.5) * n / k).astype(np.int64)])
 
 
 def read_traffic(profile: Path) -> float | None:
@@ -173,61 +163,80 @@ def _timed(fn):
 
 
 def gilmore_figures(cpu_budget_s: float | None) -> dict:
-    """1e6 bubbles, adaptive DP5(4) (rtol 1e-8, atol 1e-12), 20 us, FP64 and the
-    FP32/FP64 split; 1,024 bubbles, RK4 dt = 0.1 ns, 20 us, FP64 with R(t) every
-    1 ns; the FP64 oracle on the host for both (cpu_budget_s None: skipped),
-    with the GPU's Rmax on the stratified 4,096-bubble subset against it."""
+    """1e6 bubbles, adaptive DP5(4) (rtol 1e-8, atol 1e-12), 20 us: FP64 in the
+    fast arithmetic and in the reference's, and the FP32/FP64 split; 1,024
+    bubbles, RK4 dt = 0.1 ns, 20 us, FP64 with R(t) every 1 ns; the FP64 oracle
+    on the host for both (cpu_budget_s None: skipped), with each GPU variant's
+    Rmax on the stratified 4,096-bubble subset against it (bubbles whose Rmax
+    the oracle itself moves by > 1e-9 under a one-ulp input change are counted
+    apart: tests/golden/gilmore_conditioning.json)."""
     import numpy as np
     from bubbledynamics_amd import gilmore as gm
-    from bubbledynamics_amd._ffi import BD_GM_DP45, BD_GM_RK4, BD_GM_SCHED_SLICED, BD_GM_SPLIT
+    from bubbledynamics_amd._ffi import BD_GM_ARITH_REFERENCE, BD_GM_DP45, BD_GM_RK4, BD_GM_SCHED_SLICED, BD_GM_SPLIT
     p = gm.workload_params()
+    pref = gm.workload_params(arithmetic=BD_GM_ARITH_REFERENCE)
     R0, amp, ph, fr = gm.config3_inputs(1_000_000, 3)
     b = gm.BubbleBatch(R0, amp, ph, fr)
     gm.BubbleBatch(R0[:4096], amp[:4096], ph[:4096], fr).run(p, BD_GM_DP45, 2e-6, 1e-13)   # warm-up
-    # launch shape from the autotuner (probe: every bubble capped at 500 attempts)
-    tune, t_tune = _timed(lambda: b.autotune(p, BD_GM_DP45, 20e-6, 1e-13, rtol=1e-8, atol=1e-12))
-    # sliced schedule: passes of 128 and 2048 attempts, the rest longest-first
-    o64, t64 = _timed(lambda: b.run(p, BD_GM_DP45, 20e-6, 1e-13, rtol=1e-8, atol=1e-12,
-                                    schedule=BD_GM_SCHED_SLICED))
-    _, t64_input = _timed(lambda: b.run(p, BD_GM_DP45, 20e-6, 1e-13, rtol=1e-8, atol=1e-12))
-    s64 = int(o64["n_acc"].sum().item())
-    rej64 = int(o64["n_rej"].sum().item())
-    o32, t32 = _timed(lambda: b.run(p, BD_GM_DP45, 20e-6, 1e-13, precision=BD_GM_SPLIT, rtol=1e-8,
-                                    atol=1e-12, split_ratio=0.2, schedule=BD_GM_SCHED_SLICED))
-    s32 = int(o32["n_acc"].sum().item())
+    runs = {}
+    for name, pp, prec in (("dp45_1e6_fp64", p, 0), ("dp45_1e6_fp64_reference", pref, 0),
+                           ("dp45_1e6_split", p, BD_GM_SPLIT)):
+        # launch shape from the autotuner (probe: every bubble capped at 500
+        # attempts), then the sliced schedule (passes of 128 and 2048
+        # attempts, the rest longest-first)
+        tune, t_tune = _timed(lambda: b.autotune(pp, BD_GM_DP45, 20e-6, 1e-13, precision=prec, rtol=1e-8,
+                                                 atol=1e-12))
+        o, t = _timed(lambda: b.run(pp, BD_GM_DP45, 20e-6, 1e-13, precision=prec, rtol=1e-8, atol=1e-12,
+                                    split_ratio=0.2, schedule=BD_GM_SCHED_SLICED))
+        steps = int(o["n_acc"].sum().item())
+        runs[name] = (o, {"bubble_steps_per_s": steps / t, "accepted_steps": steps,
+                          "rejected": int(o["n_rej"].sum().item()), "seconds": t,
+                          "collapsed_frac": float((o["t_collapse"] > 0).float().mean().item()),
+                          "autotuned_shape": tune, "autotune_seconds": t_tune,
+                          "schedule": "sliced (128, 2048, rest)"})
+        log(f"{name}: {steps / t:.3e} bubble-steps/s")
+    _, t_in = _timed(lambda: b.run(p, BD_GM_DP45, 20e-6, 1e-13, rtol=1e-8, atol=1e-12))
+    runs["dp45_1e6_fp64"][1]["input_order_bubble_steps_per_s"] = runs["dp45_1e6_fp64"][1]["accepted_steps"] / t_in
+    runs["dp45_1e6_fp64"][1]["arithmetic"] = "fast (shared subexpressions, no cancellation, FSAL)"
+    runs["dp45_1e6_fp64_reference"][1]["arithmetic"] = "reference (G:40-179 expression order, 7 stages)"
+    o64 = runs["dp45_1e6_fp64"][0]
+    o32 = runs["dp45_1e6_split"][0]
     rel = (o32["Rmax"] / o64["Rmax"] - 1).abs().cpu().numpy()
-    flips = float(((o32["t_collapse"] > 0) != (o64["t_collapse"] > 0)).float().mean().item())
+    runs["dp45_1e6_split"][1]["rmax_rel_err_vs_fp64"] = {
+        "median": float(np.median(rel)), "p90": float(np.quantile(rel, 0.9)),
+        "p99": float(np.quantile(rel, 0.99)), "max": float(rel.max())}
+    runs["dp45_1e6_split"][1]["collapse_flag_flips"] = float(
+        ((o32["t_collapse"] > 0) != (o64["t_collapse"] > 0)).float().mean().item())
     R0c, ampc, phc, frc = gm.config0_inputs(1024, 0)
     bc = gm.BubbleBatch(R0c, ampc, phc, frc)
     oc, tc = _timed(lambda: bc.run(p, BD_GM_RK4, 20e-6, 1e-10, out_every=10, n_out=20001))
     sc = int(oc["n_acc"].sum().item())
-    out = {
-        "dp45_1e6_fp64": {"bubble_steps_per_s": s64 / t64, "accepted_steps": s64, "rejected": rej64,
-                          "seconds": t64, "collapsed_frac": float((o64["t_collapse"] > 0).float().mean().item()),
-                          "autotuned_shape": tune, "autotune_seconds": t_tune, "schedule": "sliced (128, 2048, rest)",
-                          "input_order_bubble_steps_per_s": s64 / t64_input, "arithmetic": "fast"},
-        "dp45_1e6_split": {"bubble_steps_per_s": s32 / t32, "accepted_steps": s32, "seconds": t32,
-                           "rmax_rel_err_vs_fp64": {"median": float(np.median(rel)),
-                                                    "p90": float(np.quantile(rel, 0.9)),
-                                                    "p99": float(np.quantile(rel, 0.99)), "max": float(rel.max())},
-                           "collapse_flag_flips": flips},
+    out = {k: v[1] for k, v in runs.items()}
+    out.update({
         "rk4_1024_fp64": {"bubble_steps_per_s": sc / tc, "steps": sc, "seconds": tc,
                           "collapsed_frac": float((oc["t_collapse"] > 0).float().mean().item())},
-        "unit": "bubble-steps/s", "tableau": "Dormand-Prince 1980",
-    }
+        "unit": "bubble-steps/s", "tableau": "Dormand-Prince 1980"})
     if cpu_budget_s is not None:
         cpu = gilmore_cpu_baseline(cpu_budget_s)
-        idx = config3_subset()
+        idx = gm.config3_subset()
         ref = np.array(cpu.pop("dp45_subset_rmax"))
         ref_c = np.array(cpu.pop("dp45_subset_collapsed"))
-        got = o64["Rmax"].cpu().numpy()[idx]
-        got_c = (o64["t_collapse"] > 0).cpu().numpy()[idx]
-        e = np.abs(got / ref - 1)
-        out["dp45_1e6_fp64"]["rmax_rel_err_vs_oracle_subset4096"] = {
-            "median": float(np.median(e)), "p99": float(np.quantile(e, 0.99)), "max": float(e.max()),
-            "collapse_flag_flips": int(np.sum(got_c != ref_c))}
+        cond = json.loads((ROOT / "tests" / "golden" / "gilmore_conditioning.json").read_text())
+        ill = set(cond["config3_subset_ulp"]["positions"])
+        well = np.array([i not in ill for i in range(len(idx))])
+        for name, (o, rec) in runs.items():
+            got = o["Rmax"].cpu().numpy()[idx]
+            got_c = (o["t_collapse"] > 0).cpu().numpy()[idx]
+            e = np.abs(got / ref - 1)
+            rec["rmax_rel_err_vs_oracle_subset4096"] = {
+                "median": float(np.median(e)), "p99": float(np.quantile(e, 0.99)), "max": float(e.max()),
+                "well_conditioned": int(well.sum()), "well_conditioned_max": float(e[well].max()),
+                "well_conditioned_over_1e-6": int(np.sum(e[well] > 1e-6)),
+                "collapse_flag_flips": int(np.sum(got_c != ref_c))}
         cpu["rk4_1024"]["gpu_x"] = (sc / tc) / cpu["rk4_1024"]["value"]
-        cpu["dp45_1e6"]["gpu_x"] = (s64 / t64) / cpu["dp45_1e6"]["value"]
+        cpu["dp45_1e6"]["gpu_x"] = runs["dp45_1e6_fp64"][1]["bubble_steps_per_s"] / cpu["dp45_1e6"]["value"]
+        cpu["dp45_1e6"]["gpu_reference_arithmetic_x"] = (runs["dp45_1e6_fp64_reference"][1]["bubble_steps_per_s"] /
+                                                         cpu["dp45_1e6"]["value"])
         out["cpu_baseline"] = cpu
     return out
 

@@ -428,7 +428,11 @@ __global__ void __launch_bounds__(256) gm_batch_kernel(GmBatchArgs a) {
             }
         }
         bool done = false;
-        const bool use32 = PREC == BD_GM_SPLIT && y[0] >= a.split_ratio * R0;
+        // BD_GM_SPLIT: FP32 right-hand sides while every active lane of the
+        // wavefront has R/R0 >= split_ratio (one precision per wavefront, so
+        // the two paths are never both executed)
+        bool use32 = PREC == BD_GM_SPLIT && y[0] >= a.split_ratio * R0;
+        if constexpr (PREC == BD_GM_SPLIT) use32 = __all(use32) != 0;
         if constexpr (METHOD == BD_GM_RK4) {
             if (PREC == BD_GM_SPLIT && use32) rk4_step(f32, s_rk * a.dt, a.dt, y);
             else rk4_step(f64, s_rk * a.dt, a.dt, y);
@@ -795,19 +799,20 @@ static int persistent_blocks(int block, int waves = 8) {
 struct GmShape { int block, chunk, waves; };
 constexpr GmShape kGmDefaultShape = {256, 1, 8};
 constexpr int kGmMaxDevices = 64;
-static GmShape g_gm_tuned[kGmMaxDevices][2][2];
-static bool g_gm_has_tuned[kGmMaxDevices][2][2];
+// indexed [device][method][precision][arithmetic]
+static GmShape g_gm_tuned[kGmMaxDevices][2][2][2];
+static bool g_gm_has_tuned[kGmMaxDevices][2][2][2];
 // guards the per-device host-side state of this file (tuned shapes, scratch
 // and counter allocations) against concurrent host threads
 static std::mutex g_gm_mutex;
 
-static GmShape gm_shape(const bd_gilmore_batch* b) {
+static GmShape gm_shape(const bd_gilmore_params* prm, const bd_gilmore_batch* b) {
     GmShape s = kGmDefaultShape;
     int dev = 0;
     std::lock_guard<std::mutex> lock(g_gm_mutex);
     if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < kGmMaxDevices &&
-        g_gm_has_tuned[dev][b->method][b->precision])
-        s = g_gm_tuned[dev][b->method][b->precision];
+        g_gm_has_tuned[dev][b->method][b->precision][prm->arithmetic])
+        s = g_gm_tuned[dev][b->method][b->precision][prm->arithmetic];
     if (b->block > 0) s.block = b->block;
     if (b->chunk > 0) s.chunk = b->chunk;
     if (b->waves > 0) s.waves = b->waves;
@@ -1262,7 +1267,7 @@ extern "C" int bd_gilmore_integrate(const bd_gilmore_params* prm, const bd_gilmo
     if (b->schedule != BD_GM_SCHED_INPUT && b->schedule != BD_GM_SCHED_SLICED) return BD_ERR_INVALID;
     if (b->slice < 0) return BD_ERR_INVALID;
     hipStream_t st = as_stream(stream);
-    const GmShape shp = gm_shape(b);
+    const GmShape shp = gm_shape(prm, b);
     if (b->schedule == BD_GM_SCHED_INPUT) return gm_launch(prm, b, workspace, shp, st);
     // Passes of growing slices (S, 16 S, then to the end): every pass runs the
     // bubbles still unfinished, largest predicted remaining work first, so a
@@ -1360,8 +1365,8 @@ extern "C" int bd_gilmore_autotune(const bd_gilmore_params* prm, const bd_gilmor
     }
     {
         std::lock_guard<std::mutex> lock(g_gm_mutex);
-        g_gm_tuned[dev][b.method][b.precision] = best;
-        g_gm_has_tuned[dev][b.method][b.precision] = true;
+        g_gm_tuned[dev][b.method][b.precision][prm->arithmetic] = best;
+        g_gm_has_tuned[dev][b.method][b.precision][prm->arithmetic] = true;
     }
     if (best_block) *best_block = best.block;
     if (best_chunk) *best_chunk = best.chunk;

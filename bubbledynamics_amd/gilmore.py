@@ -286,6 +286,14 @@ def config3_inputs(n: int = 1_000_000, seed: int = 3):
     return R0, amp, ph, (1e6, 2e6, 3e6)
 
 
+def config3_subset(n: int = 1_000_000, k: int = 4096) -> np.ndarray:
+    """Indices (increasing) of k of the n config-3 bubbles, stratified in R0:
+    the bubbles at ranks (j + 1/2) n / k of the R0 order."""
+    R0, _, _, _ = config3_inputs(n, 3)
+    order = np.argsort(R0, kind="stable")
+    return np.sort(order[((np.arange(k) + 0.5) * n / k).astype(np.int64)])
+
+
 def _splitmix_vec(seed: int, n: int) -> np.ndarray:
     """Vectorised SplitMix64 (identical stream to seeded_uniforms)."""
     with np.errstate(over="ignore"):

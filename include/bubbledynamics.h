@@ -348,7 +348,9 @@ int bd_gilmore_threshold_sweep(const bd_gilmore_params* prm, double* results_A,
 #define BD_GM_DP45  1   /* adaptive Dormand-Prince 5(4), controller of G:212-318    */
 
 #define BD_GM_FP64   0  /* all arithmetic in FP64                                    */
-#define BD_GM_SPLIT  1  /* FP64 state/error norm; FP32 right-hand side while R/R0 >= split_ratio */
+#define BD_GM_SPLIT  1  /* FP64 state/error norm; FP32 right-hand side while every bubble of the
+                           wavefront has R/R0 >= split_ratio (results then depend on which
+                           bubbles share a wavefront, i.e. on the launch shape)          */
 
 /* One batch of independent bubbles driven by
  * p(t) = sum_k amp[3i+k] sin(2 pi freq[k] t + phase[3i+k]).
@@ -468,7 +470,7 @@ int bd_gilmore_sweep_write(const char* path, const double* fr_combo, int n_combo
  * the reference): times block in {64,128,256} x waves per SIMD in {2,4,8}
  * (and bubbles per grab in {1,4} at 8 waves) on the first `sample` bubbles of
  * the batch (0 = up to 2^20), each capped at probe_steps attempts (0 = 500),
- * and records the fastest for this device, method and precision; later
+ * and records the fastest for this device, method, precision and arithmetic; later
  * integrate calls use it for block / chunk / waves left at 0.  Returns the
  * choice and its rate (bubble-steps/s).  Allocates scratch; synchronous. */
 int bd_gilmore_autotune(const bd_gilmore_params* prm, const bd_gilmore_batch* batch, int sample,

@@ -121,25 +121,43 @@ def test_dp45_batch_reference_arithmetic_matches_oracle(orc, tab):
     np.testing.assert_array_equal(host(out["t_collapse"]) > 0, ref["t_collapse"] > 0)
 
 
-def test_split_precision_rmax_error():
-    # FP32 right-hand sides while R/R0 >= 0.2, FP64 below; Rmax against all-FP64
+def test_split_precision_rmax_error(orc):
+    # FP32 right-hand sides while R/R0 >= 0.2 (per wavefront), FP64 below, on
+    # the stratified config-3 subset over 20 us: Rmax against the FP64 oracle
+    # and against the all-FP64 kernel.  rtol 1e-8 is below FP32's epsilon and
+    # atol 1e-12 on dR/dt (SI) is far below the FP32 rounding of d2R/dt2 near
+    # rest, so the split does not reach the 1e-6 target: the bound here is
+    # what it achieves on the bubbles the oracle can pin (one-ulp inputs move
+    # Rmax by <= 1e-9), and all-FP64 is the configuration's answer.
+    import json
+    from pathlib import Path
     p = gm.workload_params()
-    R0, amp, ph, fr = gm.config3_inputs(2048, 3)
+    idx = gm.config3_subset()
+    R0, amp, ph, fr = gm.config3_inputs(1_000_000, 3)
+    R0, amp, ph = R0[idx], amp[idx], ph[idx]
     T = 20e-6
     b = gm.BubbleBatch(R0, amp, ph, fr)
     o64 = b.run(p, BD_GM_DP45, T, 1e-13, rtol=1e-8, atol=1e-12)
     o32 = b.run(p, BD_GM_DP45, T, 1e-13, precision=BD_GM_SPLIT, rtol=1e-8, atol=1e-12)
     torch.cuda.synchronize()
-    rel = np.abs(host(o32["Rmax"]) / host(o64["Rmax"]) - 1)
-    print(f"split Rmax rel err: max {rel.max():.3e} median {np.median(rel):.3e} "
-          f"p99 {np.quantile(rel, 0.99):.3e}; steps fp64 {host(o64['n_acc']).sum()} split {host(o32['n_acc']).sum()}")
-    # FP32 stages put ~1e-7 relative noise into the DP5(4) error estimate, which
-    # is at the rtol = 1e-8 scale; bubbles whose collapse (dR/dt <= -c) is
-    # marginal can stop in one precision and not in the other
-    flips = np.mean((host(o32["t_collapse"]) > 0) != (host(o64["t_collapse"]) > 0))
-    print(f"collapse-flag flips: {flips:.4f}")
-    assert np.median(rel) <= 2e-5
-    assert flips <= 0.05
+    ref = orc.gilmore_integrate(p, BD_GM_DP45, R0, amp, ph, fr, T, 1e-13, rtol=1e-8, atol=1e-12, nthreads=16)
+    fix = json.loads((Path(__file__).parent / "golden" / "gilmore_conditioning.json").read_text())
+    ill = set(fix["config3_subset_ulp"]["positions"])
+    well = np.array([i not in ill for i in range(len(idx))])
+    rel = np.abs(host(o32["Rmax"]) / ref["Rmax"] - 1)
+    rel64 = np.abs(host(o32["Rmax"]) / host(o64["Rmax"]) - 1)
+    flips = host(o32["t_collapse"] > 0) != (ref["t_collapse"] > 0)
+    print(f"split vs oracle: median {np.median(rel):.3e} p99 {np.quantile(rel, 0.99):.3e} max {rel.max():.3e}; "
+          f"well-conditioned max {rel[well].max():.3e}, > 1e-6: {int(np.sum(rel[well] > 1e-6))}/{well.sum()}; "
+          f"vs all-FP64 median {np.median(rel64):.3e}; collapse flips {int(flips.sum())}; accepted steps "
+          f"split {host(o32['n_acc']).sum()} FP64 {host(o64['n_acc']).sum()}")
+    keep = well & ~flips
+    print(f"split: well-conditioned bubbles without a collapse flip: max {rel[keep].max():.3e}, "
+          f"p99 {np.quantile(rel[keep], 0.99):.3e}")
+    # measured: median 3.3e-6, 14 flips in 4,096; the bounds guard that level
+    assert np.median(rel) <= 1e-5
+    assert np.quantile(rel[keep], 0.99) <= 1e-2
+    assert np.mean(flips) <= 0.01
 
 
 def test_queue_scheduling_does_not_change_results():
