@@ -511,6 +511,8 @@ def main() -> int:
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the Gilmore, adjoint and coupled bioheat figures")
+    ap.add_argument("--figure", choices=("gilmore", "adjoint", "coupled"), default=None,
+                    help="run only that auxiliary workload and print its JSON (for profiling)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first copy a buffer of this size 3 times (HBM counter calibration)")
     ap.add_argument("--reference-scenario", action="store_true",
@@ -564,6 +566,11 @@ def main() -> int:
         res = reference_scale(args.shifts, args.compare, args.slices or None, args.thermal_dtype == "f64",
                               args.slices_npy or None)
         print(json.dumps({"workload": "reference_scenario", "n_gpus": 1, **res}), flush=True)
+        return 0
+    if args.figure:
+        fn = {"gilmore": lambda: gilmore_figures(None), "adjoint": adjoint_figures,
+              "coupled": coupled_figures}[args.figure]
+        print(json.dumps({"figure": args.figure, "n_gpus": 1, **fn()}), flush=True)
         return 0
     if args.calib_copy_mb:
         src = torch.ones(args.calib_copy_mb * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
