@@ -309,6 +309,16 @@ __global__ void energy_finalize_kernel(const double* epart, double* energy, int 
 constexpr int kDefaultVariant = 0;   // 64 x 4 tile, one row per thread
 constexpr int kDeepVariant = 2;      // 64 x 4 tile, two rows per thread, plane prefetch
 constexpr int kDefaultBlocks = 1024;
+// Tiles bd_westervelt_plan_prepare times on the bound state and chooses from
+// (the first entry is the static choice of launch_shape's rule).  One MI355X,
+// FP32 tone burst, us per step, variants 2 / 10 / 13 (64 x 4 tile; two rows,
+// one plane in flight / one row, two planes / one row, three planes): 128^3
+// 29.0 / 24.7 / 24.3, 160^3 43.9 / 45.7 / 44.0, 192^3 71.4 / 68.0 / 71.4,
+// 224^3 122.6 / 125.7 / 122.7, 256^3 166.0 / 167.0 / 166.0; FP64 192^3
+// variants 0 / 10 / 13: 158.3 / 149.4 / 159.6.  No geometric rule separates
+// them, so the plan measures.
+constexpr int kTuneF32[] = {kDeepVariant, 10, 13};
+constexpr int kTuneF64[] = {kDefaultVariant, 10};
 
 // The 12-field state does not fit in the last-level cache: the step kernel
 // then loads the history levels with the non-temporal policy.
@@ -403,10 +413,10 @@ template <typename T, int V>
 static void launch_variant(const WvMarchArgs<T>& a, int step_arg, hipStream_t st) {
     constexpr WvVariant v = kWvVariants[V];
     if (a.nt)
-        hipLaunchKernelGGL((wv_march<T, v.tx, v.ty, v.ry, (bool)v.pf, true>), dim3(a.g.nblocks),
+        hipLaunchKernelGGL((wv_march<T, v.tx, v.ty, v.ry, v.pf, true>), dim3(a.g.nblocks),
                            dim3(v.tx * v.ty), 0, st, a, step_arg);
     else
-        hipLaunchKernelGGL((wv_march<T, v.tx, v.ty, v.ry, (bool)v.pf, false>), dim3(a.g.nblocks),
+        hipLaunchKernelGGL((wv_march<T, v.tx, v.ty, v.ry, v.pf, false>), dim3(a.g.nblocks),
                            dim3(v.tx * v.ty), 0, st, a, step_arg);
 }
 
@@ -422,6 +432,11 @@ static int launch_march(const WvMarchArgs<T>& a, int variant, int step_arg, hipS
         case 6: launch_variant<T, 6>(a, step_arg, st); break;
         case 7: launch_variant<T, 7>(a, step_arg, st); break;
         case 8: launch_variant<T, 8>(a, step_arg, st); break;
+        case 9: launch_variant<T, 9>(a, step_arg, st); break;
+        case 10: launch_variant<T, 10>(a, step_arg, st); break;
+        case 11: launch_variant<T, 11>(a, step_arg, st); break;
+        case 12: launch_variant<T, 12>(a, step_arg, st); break;
+        case 13: launch_variant<T, 13>(a, step_arg, st); break;
         default: return BD_ERR_INVALID;
     }
     BD_LAUNCH_CHECK();
@@ -685,6 +700,9 @@ struct bd_westervelt_plan {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
     int device = 0;
+    int target_blocks = 0;              // workgroup target of make_geom
+    int epart_blocks = 0;               // per-step capacity of d_epart (largest candidate tile)
+    bool tuned = false;                 // the step tile was chosen by measurement (prepare)
 };
 
 extern "C" size_t bd_westervelt_state_elems(int Nxy, int Nz) {
@@ -704,7 +722,11 @@ extern "C" int bd_westervelt_plan_create(const bd_westervelt_params* prm, int dt
     pl->prm = *prm; pl->dtype = dtype; pl->Nxy = Nxy; pl->Nz = Nz; pl->max_steps = max_steps;
     int blocks = 0;
     launch_shape(prm, Nxy, Nz, dtype == BD_F64, &pl->variant, &blocks);
+    pl->target_blocks = blocks;
     pl->g = make_geom(prm, Nxy, Nz, pl->variant, blocks);
+    pl->epart_blocks = pl->g.nblocks;
+    for (int c : kTuneF32) pl->epart_blocks = std::max(pl->epart_blocks, make_geom(prm, Nxy, Nz, c, blocks).nblocks);
+    for (int c : kTuneF64) pl->epart_blocks = std::max(pl->epart_blocks, make_geom(prm, Nxy, Nz, c, blocks).nblocks);
     const int lo = prm->full_domain ? 3 : 0;
     pl->focus_idx = -1;
     if (fx >= 0) {
@@ -712,7 +734,7 @@ extern "C" int bd_westervelt_plan_create(const bd_westervelt_params* prm, int dt
         pl->focus_idx = (long long)fz * Nxy * Nxy + (long long)fy * Nxy + fx;
     }
     if (hipGetDevice(&pl->device) != hipSuccess ||
-        hipMalloc(&pl->d_epart, sizeof(double) * 2 * (size_t)pl->g.nblocks * max_steps) != hipSuccess ||
+        hipMalloc(&pl->d_epart, sizeof(double) * 2 * (size_t)pl->epart_blocks * max_steps) != hipSuccess ||
         hipMalloc(&pl->d_time, sizeof(double) * ((size_t)max_steps + 1)) != hipSuccess ||
         hipMalloc(&pl->d_step, 2 * sizeof(int)) != hipSuccess) {
         bd_westervelt_plan_destroy(pl);
@@ -724,7 +746,7 @@ extern "C" int bd_westervelt_plan_create(const bd_westervelt_params* prm, int dt
     double t = 0.0;
     for (int k = 0; k <= max_steps; ++k) { tt[k] = t; t = t + prm->dt; }
     if (hipMemcpy(pl->d_time, tt.data(), sizeof(double) * tt.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(pl->d_epart, 0, sizeof(double) * 2 * (size_t)pl->g.nblocks * max_steps) != hipSuccess) {
+        hipMemset(pl->d_epart, 0, sizeof(double) * 2 * (size_t)pl->epart_blocks * max_steps) != hipSuccess) {
         bd_westervelt_plan_destroy(pl);
         return BD_ERR_HIP;
     }
@@ -781,25 +803,19 @@ static int pmax_window(int n, int s0) {
     return (1 << std::min(6, n + 1 - s0)) - 1;
 }
 
-// pmax = max(pmax, p^l) for the levels l0..l1 (oldest first) left after the
-// last window of a run.
-template <typename T>
-__global__ void pmax_fold_kernel(T* __restrict__ pmax, const T* __restrict__ S, size_t N, int l0, int l1) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
-        const T old = pmax[i];
-        T m = old;
-        for (int l = l0; l <= l1; ++l) {
-            const T v = S[(size_t)(l % 6) * N + i];
-            if (v > m) m = v;
-        }
-        if (m > old) pmax[i] = m;
+// The same for a run ending at step `end`: its last step also takes the
+// levels after the run's last window (at most five, all held by that step),
+// so pmax is exact when the run's work is done, with no extra pass.
+static int pmax_mask(int n, int s0, int end) {
+    int m = pmax_window(n, s0);
+    if (n == end - 1) {
+        const int last = 6 * (end / 6);
+        const int l0 = (last >= s0 + 1 ? last : s0) + 1;
+        if (l0 <= end) m |= (1 << (end - l0 + 1)) - 1;
     }
+    return m;
 }
 
-// Captures and instantiates the pair of 6-step graphs if they do not exist
-// yet.  Graph k reads step counter k and its last kernel writes counter 1-k
-// (= counter k + 6) for the next replay, so no kernel ever writes the counter
-// it reads.  Capture runs on a stream of its own; nothing is launched.
 template <typename T>
 static int plan_capture_t(bd_westervelt_plan* pl) {
     for (int k = 0; k < 2; ++k) {
@@ -831,7 +847,7 @@ static int plan_run_t(bd_westervelt_plan* pl, int n_steps, int use_graph, hipStr
     const int end = n + n_steps;
     // eager steps until the ring phase is 0 mod 6
     while (n < end && (!use_graph || n % 6 != 0)) {
-        int rc = plan_step<T>(pl, n, nullptr, n, st, pmax_window(n, s0));
+        int rc = plan_step<T>(pl, n, nullptr, n, st, pmax_mask(n, s0, end));
         if (rc) return rc;
         ++n;
         if (!use_graph) continue;
@@ -847,23 +863,11 @@ static int plan_run_t(bd_westervelt_plan* pl, int n_steps, int use_graph, hipStr
         }
     }
     while (n < end) {
-        int rc = plan_step<T>(pl, n, nullptr, n, st, pmax_window(n, s0));
+        int rc = plan_step<T>(pl, n, nullptr, n, st, pmax_mask(n, s0, end));
         if (rc) return rc;
         ++n;
     }
     pl->steps = n;
-    // levels after the run's last window
-    if (pl->pmax) {
-        const int last = 6 * (end / 6);
-        const int l0 = (last >= s0 + 1 ? last : s0) + 1;
-        if (l0 <= end) {
-            const size_t N = (size_t)pl->Nxy * pl->Nxy * pl->Nz;
-            const int blocks = (int)std::min<size_t>((N + 255) / 256, 8192);
-            hipLaunchKernelGGL((pmax_fold_kernel<T>), dim3(blocks), dim3(256), 0, st, (T*)pl->pmax,
-                               (const T*)pl->state, N, l0, end);
-            BD_LAUNCH_CHECK();
-        }
-    }
     return BD_OK;
 }
 
@@ -878,17 +882,13 @@ extern "C" int bd_westervelt_plan_run(bd_westervelt_plan* pl, int n_steps, int u
     return plan_run_t<double>(pl, n_steps, use_graph, as_stream(stream));
 }
 
-extern "C" int bd_westervelt_plan_prepare(bd_westervelt_plan* pl) {
-    if (!pl || !pl->state) return BD_ERR_INVALID;
-    int dev = 0;
-    BD_HIP_CHECK(hipGetDevice(&dev));
-    if (dev != pl->device) return BD_ERR_INVALID;
-    if (pl->dtype == BD_F32) return plan_capture_t<float>(pl);
-    return plan_capture_t<double>(pl);
-}
 
+// Launches the step kernel of the next step again with tile `variant` on
+// geometry g (state not advanced; it rewrites the next levels, the step's
+// energy partials and focus sample, which the real step overwrites, and
+// with use_pmax the peak-pressure field).
 template <typename T>
-static int plan_rerun_step_t(bd_westervelt_plan* pl, hipStream_t st) {
+static int plan_rerun_step_t(bd_westervelt_plan* pl, hipStream_t st, int variant, const WvGeom& g, bool use_pmax) {
     const int n = pl->steps;
     const size_t N = (size_t)pl->Nxy * pl->Nxy * pl->Nz;
     T* S = (T*)pl->state;
@@ -899,24 +899,81 @@ static int plan_rerun_step_t(bd_westervelt_plan* pl, hipStream_t st) {
     a.pn = P(n + 1); a.p = P(n); a.pm1 = P(n - 1); a.pm2 = P(n - 2); a.pm3 = P(n - 3); a.pm4 = P(n - 4);
     a.r1 = R1(n); a.r1m1 = R1(n - 1); a.r1m2 = R1(n - 2);
     a.r2 = R2(n); a.r2m1 = R2(n - 1); a.r2m2 = R2(n - 2);
-    a.pmax = (T*)pl->pmax;
+    a.pmax = use_pmax ? (T*)pl->pmax : nullptr;
     a.pmax_mask = (n + 1) % 6 == 0 ? 63 : 0;      // the window of the step inside a replay
     a.epart = pl->energy ? pl->d_epart : nullptr;
     a.focus = pl->focus;
     a.d_step = nullptr;
     a.d_step_next = nullptr;
     a.focus_idx = pl->focus_idx;
-    a.g = pl->g;
+    a.g = g;
     a.c = make_coef<T>(&pl->prm);
     set_fused_z<T>(a, &pl->prm, 1, pl->amp, pl->phase);
     a.t_next = pl->h_time[(size_t)n + 1];
-    return launch_march<T>(a, pl->variant, n, st);
+    return launch_march<T>(a, variant, n, st);
 }
 
 extern "C" int bd_westervelt_plan_rerun_step(bd_westervelt_plan* pl, void* stream) {
     if (!pl || !pl->state || pl->steps >= pl->max_steps) return BD_ERR_INVALID;
-    if (pl->dtype == BD_F32) return plan_rerun_step_t<float>(pl, as_stream(stream));
-    return plan_rerun_step_t<double>(pl, as_stream(stream));
+    if (pl->dtype == BD_F32) return plan_rerun_step_t<float>(pl, as_stream(stream), pl->variant, pl->g, true);
+    return plan_rerun_step_t<double>(pl, as_stream(stream), pl->variant, pl->g, true);
+}
+
+// Chooses the step tile by measurement: each candidate relaunches the next
+// step (2 untimed + 12 timed launches, pmax untouched, nothing the real step
+// does not overwrite) on a private stream; the fastest mean is kept.  Runs
+// once per plan, before its first step; a BD_WV_VARIANT override skips it.
+template <typename T>
+static int plan_tune_t(bd_westervelt_plan* pl) {
+    pl->tuned = true;
+    if (std::getenv("BD_WV_VARIANT") || pl->steps != 0 || pl->steps >= pl->max_steps) return BD_OK;
+    const bool f32 = std::is_same<T, float>::value;
+    const int* cands = f32 ? kTuneF32 : kTuneF64;
+    const int nc = f32 ? (int)(sizeof(kTuneF32) / sizeof(int)) : (int)(sizeof(kTuneF64) / sizeof(int));
+    struct Res {
+        hipStream_t st = nullptr; hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Res() {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (st) (void)hipStreamDestroy(st);
+        }
+    } r;
+    BD_HIP_CHECK(hipStreamCreateWithFlags(&r.st, hipStreamNonBlocking));
+    BD_HIP_CHECK(hipEventCreate(&r.e0));
+    BD_HIP_CHECK(hipEventCreate(&r.e1));
+    constexpr int kWarm = 2, kTimed = 12;
+    int best = pl->variant;
+    float best_ms = 1e30f;
+    for (int i = 0; i < nc; ++i) {
+        const WvGeom g = make_geom(&pl->prm, pl->Nxy, pl->Nz, cands[i], pl->target_blocks);
+        if (g.nblocks > pl->epart_blocks) continue;
+        for (int k = 0; k < kWarm + kTimed; ++k) {
+            if (k == kWarm) BD_HIP_CHECK(hipEventRecord(r.e0, r.st));
+            const int rc = plan_rerun_step_t<T>(pl, r.st, cands[i], g, false);
+            if (rc) return rc;
+        }
+        BD_HIP_CHECK(hipEventRecord(r.e1, r.st));
+        BD_HIP_CHECK(hipEventSynchronize(r.e1));
+        float ms = 0.f;
+        BD_HIP_CHECK(hipEventElapsedTime(&ms, r.e0, r.e1));
+        if (ms < best_ms) { best_ms = ms; best = cands[i]; }
+    }
+    pl->variant = best;
+    pl->g = make_geom(&pl->prm, pl->Nxy, pl->Nz, best, pl->target_blocks);
+    return BD_OK;
+}
+
+extern "C" int bd_westervelt_plan_prepare(bd_westervelt_plan* pl) {
+    if (!pl || !pl->state) return BD_ERR_INVALID;
+    int dev = 0;
+    BD_HIP_CHECK(hipGetDevice(&dev));
+    if (dev != pl->device) return BD_ERR_INVALID;
+    if (pl->dtype == BD_F32) {
+        if (!pl->tuned && !pl->gexec[0]) { const int rc = plan_tune_t<float>(pl); if (rc) return rc; }
+        return plan_capture_t<float>(pl);
+    }
+    if (!pl->tuned && !pl->gexec[0]) { const int rc = plan_tune_t<double>(pl); if (rc) return rc; }
+    return plan_capture_t<double>(pl);
 }
 
 extern "C" long long bd_westervelt_plan_interior_cells(const bd_westervelt_plan* pl) {

@@ -142,7 +142,7 @@ struct PlaneIn {
     T r1m1[RY], r1m2[RY], r2m1[RY], r2m2[RY], pmx[RY];
 };
 
-template <typename T, int TX, int TY, int RY, bool PF, bool NTL>
+template <typename T, int TX, int TY, int RY, int PF, bool NTL>
 __global__ void __launch_bounds__(TX * TY)
 wv_march(WvMarchArgs<T> a, int step_arg) {
     constexpr int NT = TX * TY;
@@ -292,11 +292,15 @@ wv_march(WvMarchArgs<T> a, int step_arg) {
 #pragma unroll
     for (int r = 0; r < RY; ++r) lastp[r] = T(0);
 
-    PlaneIn<T, RY> cur;
+    // PF planes of loads in flight ahead of the plane being computed
+    static_assert(PF >= 0 && PF <= 3, "prefetch depth 0..3");
+    PlaneIn<T, RY> cur, nx1, nx2;
     load_plane(z0, cur);
+    if (PF >= 2 && z0 + 1 < z1) load_plane(z0 + 1, nx1);
+    if (PF >= 3 && z0 + 2 < z1) load_plane(z0 + 2, nx2);
     for (int z = z0; z < z1; ++z) {
         PlaneIn<T, RY> nxt;
-        if (PF && z + 1 < z1) load_plane(z + 1, nxt);
+        if (PF >= 1 && z + PF < z1) load_plane(z + PF, nxt);
         const int b = (z - z0) & 1;
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
@@ -412,8 +416,15 @@ wv_march(WvMarchArgs<T> a, int step_arg) {
 #pragma unroll
             for (int k = 0; k < 6; ++k) q[r][k] = q[r][k + 1];
         }
-        if (PF) {
+        if (PF == 1) {
             cur = nxt;
+        } else if (PF == 2) {
+            cur = nx1;
+            nx1 = nxt;
+        } else if (PF == 3) {
+            cur = nx1;
+            nx1 = nx2;
+            nx2 = nxt;
         } else if (z + 1 < z1) {
             load_plane(z + 1, cur);
         }
@@ -581,10 +592,12 @@ wv_march(WvMarchArgs<T> a, int step_arg) {
 
 // Tile shapes the interior kernel is instantiated for.
 struct WvVariant { int tx, ty, ry, pf; };
+// pf: planes of loads in flight ahead of the plane being computed (0..3)
 constexpr WvVariant kWvVariants[] = {
     {64, 4, 1, 0}, {64, 4, 1, 1}, {64, 4, 2, 1}, {64, 8, 1, 1},
     {64, 8, 2, 1}, {64, 4, 4, 1}, {64, 8, 2, 0}, {32, 8, 2, 1},
-    {128, 4, 1, 0},
+    {128, 4, 1, 0}, {64, 4, 2, 2}, {64, 4, 1, 2}, {64, 8, 1, 2},
+    {128, 4, 1, 2}, {64, 4, 1, 3},
 };
 constexpr int kWvNumVariants = sizeof(kWvVariants) / sizeof(kWvVariants[0]);
 

@@ -158,14 +158,20 @@ int bd_westervelt_plan_bind(bd_westervelt_plan* plan, void* state, void* pmax, d
  * replays a captured 6-step hipGraph (the ring period).  pmax is the running
  * maximum over every level this plan produced once the call's work on the
  * stream is done (inside a call the interior cells are updated once every 6
- * steps over the six levels a step holds, and the call ends with a fold of
- * the remaining levels); levels present before a call (e.g. restored after
- * bd_westervelt_plan_seek) are not taken into pmax. */
+ * steps over the six levels a step holds, and the call's last step also takes
+ * the levels after the last such window); levels present before a call (e.g.
+ * restored after bd_westervelt_plan_seek) are not taken into pmax. */
 int bd_westervelt_plan_run(bd_westervelt_plan* plan, int n_steps, int use_graph, void* stream);
-/* Captures and instantiates the plan's pair of 6-step hipGraphs now (after
- * bind; a later bind drops them), so that no later bd_westervelt_plan_run
- * with use_graph=1 spends host time on graph construction.  Optional: run
- * builds them on first use otherwise.  Launches nothing; synchronous. */
+/* Chooses the step kernel's tile by measurement, then captures and
+ * instantiates the plan's pair of 6-step hipGraphs (after bind; a later bind
+ * drops the graphs), so that no later bd_westervelt_plan_run with
+ * use_graph=1 spends host time on graph construction.  The tile choice runs
+ * once, before the plan's first step: each candidate tile relaunches the
+ * next step a few times on a private stream (writing only what that step
+ * will overwrite; pmax untouched) and the fastest is kept -- results do not
+ * depend on the tile beyond FP rounding order of the energy partials.
+ * Optional: without it the plan keeps a static tile and run builds the
+ * graphs on first use.  Synchronous. */
 int bd_westervelt_plan_prepare(bd_westervelt_plan* plan);
 /* Reduces the per-block energy partials of all steps run so far into the
  * bound energy buffer. */

@@ -158,10 +158,12 @@ def test_plan_run_parity_reference_physics(orc, dtype):
 
 
 def test_graph_replay_equals_eager_bitwise():
+    # (static tile: a tile measured per plan could differ between the plans and
+    # regroup the energy partials)
     n, nz, steps = 40, 48, 40
     p = presets.reference_scaled(n, nz)
-    a = WesterveltRun(p, n, nz, dtype=torch.float32, max_steps=steps, focus=(0, 0, 30))
-    b = WesterveltRun(p, n, nz, dtype=torch.float32, max_steps=steps, focus=(0, 0, 30))
+    a = WesterveltRun(p, n, nz, dtype=torch.float32, max_steps=steps, focus=(0, 0, 30), prepare_graphs=False)
+    b = WesterveltRun(p, n, nz, dtype=torch.float32, max_steps=steps, focus=(0, 0, 30), prepare_graphs=False)
     a.run(steps, use_graph=False)
     b.run(5, use_graph=True)      # eager head, then graph replays, then eager tail
     b.run(steps - 5, use_graph=True)
@@ -172,7 +174,7 @@ def test_graph_replay_equals_eager_bitwise():
     assert torch.equal(a.energy, b.energy)
     assert torch.equal(a.focus, b.focus)
     # determinism: a second identical run gives identical bits
-    c = WesterveltRun(p, n, nz, dtype=torch.float32, max_steps=steps, focus=(0, 0, 30))
+    c = WesterveltRun(p, n, nz, dtype=torch.float32, max_steps=steps, focus=(0, 0, 30), prepare_graphs=False)
     c.run(steps)
     c.finalize()
     torch.cuda.synchronize()
@@ -234,7 +236,8 @@ def test_plan_faces_at_tile_edges(orc, n, full):
 # tile shapes the step kernel is instantiated for: (columns, thread rows,
 # rows per thread, plane prefetch), selected with BD_WV_VARIANT
 TILE_SHAPES = [(64, 4, 1, 0), (64, 4, 1, 1), (64, 4, 2, 1), (64, 8, 1, 1), (64, 8, 2, 1),
-               (64, 4, 4, 1), (64, 8, 2, 0), (32, 8, 2, 1), (128, 4, 1, 0)]
+               (64, 4, 4, 1), (64, 8, 2, 0), (32, 8, 2, 1), (128, 4, 1, 0), (64, 4, 2, 2),
+               (64, 4, 1, 2), (64, 8, 1, 2), (128, 4, 1, 2), (64, 4, 1, 3)]
 _TILE_ORACLE = {}
 
 
@@ -307,21 +310,47 @@ def test_tile_shapes_fp32_bitwise_equal(monkeypatch, variant):
 
 
 def test_default_shape_rule(monkeypatch):
-    # FP32 takes the two-row prefetching tile unless it would leave fewer than
-    # 3/4 of the one-row tile's workgroups; FP64 keeps the one-row tile; the
-    # history loads are non-temporal once the 12-field state exceeds 256 MB
+    # without the measured choice of bd_westervelt_plan_prepare: FP32 takes the
+    # two-row prefetching tile unless it would leave fewer than 3/4 of the
+    # one-row tile's workgroups; FP64 keeps the one-row tile; the history
+    # loads are non-temporal once the 12-field state exceeds 256 MB
     monkeypatch.delenv("BD_WV_VARIANT", raising=False)
     monkeypatch.delenv("BD_WV_BLOCKS", raising=False)
     expect = {(192, torch.float32): (2, 1, 1), (128, torch.float32): (1, 0, 0),
               (160, torch.float32): (2, 1, 0), (192, torch.float64): (1, 0, 1)}
     for (n, dt), (rows, pf, nt) in expect.items():
         p = presets.tone_burst_192(n)
-        run = WesterveltRun(p, n, n, dtype=dt, max_steps=1, pmax=False, energy=False)
+        run = WesterveltRun(p, n, n, dtype=dt, max_steps=1, pmax=False, energy=False, prepare_graphs=False)
         sh = run.shape
         assert (sh["rows"], sh["prefetch"], sh["nontemporal"]) == (rows, pf, nt), (n, dt, sh)
         assert sh["tile_x"] == 64 and 768 <= sh["workgroups"] <= 1024
         run.close()
         del run
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_prepare_chooses_a_measured_tile(monkeypatch, dtype):
+    # bd_westervelt_plan_prepare times the candidate tiles on the bound state
+    # before the first step and keeps the fastest; its side effects are
+    # overwritten by the first step, so fields, pmax and focus equal those of
+    # the static tile bitwise (energy to the partials' grouping)
+    monkeypatch.delenv("BD_WV_VARIANT", raising=False)
+    monkeypatch.delenv("BD_WV_BLOCKS", raising=False)
+    cands = {torch.float32: (2, 10, 13), torch.float64: (0, 10)}[dtype]
+    n, nz, steps = 67, 40, 60
+    p, focus, _ = _tile_case(None, n, 0, nz, steps)
+    a = WesterveltRun(p, n, nz, dtype=dtype, max_steps=steps, focus=focus)             # measured tile
+    b = WesterveltRun(p, n, nz, dtype=dtype, max_steps=steps, focus=focus, prepare_graphs=False)
+    sh = a.shape
+    assert (sh["tile_x"], sh["tile_y"], sh["rows"], sh["prefetch"]) in [TILE_SHAPES[c] for c in cands]
+    a.run(steps)
+    b.run(steps)
+    a.finalize(); b.finalize()
+    torch.cuda.synchronize()
+    assert np.abs(host(a.pressure(0))).max() > 0
+    assert torch.equal(a.state, b.state) and torch.equal(a.pmax, b.pmax) and torch.equal(a.focus, b.focus)
+    ea, eb = host(a.energy), host(b.energy)
+    assert np.abs(ea - eb).max() <= 1e-6 * np.abs(eb).max()
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
