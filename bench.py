@@ -139,13 +139,13 @@ def gilmore_cpu_baseline(budget_s: float) -> dict:
             "dp45_subset_rmax": r3["Rmax"].tolist(), "dp45_subset_collapsed": (r3["t_collapse"] > 0).tolist()}
 
 
-def read_traffic(profile: Path) -> float | None:
-    """HBM bytes per launch of the step kernel from the committed counter
-    summary (line 'hbm_bytes_per_launch: N'), or None."""
+def read_traffic(profile: Path, kernel: str) -> float | None:
+    """HBM bytes per launch of the step kernel `kernel` from the committed
+    counter summary (line 'hbm_bytes_per_launch <kernel>: N'), or None."""
     try:
         for line in profile.read_text().splitlines():
-            if line.startswith("hbm_bytes_per_launch:"):
-                return float(line.split(":", 1)[1])
+            if line.startswith(f"hbm_bytes_per_launch {kernel}:"):
+                return float(line.rsplit(":", 1)[1])
     except OSError:
         pass
     return None
@@ -567,11 +567,6 @@ def main() -> int:
                               args.slices_npy or None)
         print(json.dumps({"workload": "reference_scenario", "n_gpus": 1, **res}), flush=True)
         return 0
-    if args.figure:
-        fn = {"gilmore": lambda: gilmore_figures(None), "adjoint": adjoint_figures,
-              "coupled": coupled_figures}[args.figure]
-        print(json.dumps({"figure": args.figure, "n_gpus": 1, **fn()}), flush=True)
-        return 0
     if args.calib_copy_mb:
         src = torch.ones(args.calib_copy_mb * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -579,6 +574,11 @@ def main() -> int:
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.figure:
+        fn = {"gilmore": lambda: gilmore_figures(None), "adjoint": adjoint_figures,
+              "coupled": coupled_figures}[args.figure]
+        print(json.dumps({"figure": args.figure, "n_gpus": 1, **fn()}), flush=True)
+        return 0
     n, K, W = args.n, args.steps, args.warmup
     reps = max(5, args.reps)
     p = presets.tone_burst_192(n)
@@ -636,8 +636,8 @@ def main() -> int:
     achieved = alg_bytes / (step_ms / 1e3)
     log(f"step kernel {step_ms * 1e3:.2f} us per step in the timed region ({k_ms * 1e3:.2f} us relaunched alone) "
         f"-> {achieved / 1e9:.0f} GB/s algorithmic; shape {shape}")
-    kname = "wv_march<float,%d,%d,%d,%s,%s>" % (
-        shape["tile_x"], shape["tile_y"], shape["rows"], "true" if shape["prefetch"] else "false",
+    kname = "wv_march<float, %d, %d, %d, %d, %s>" % (
+        shape["tile_x"], shape["tile_y"], shape["rows"], shape["prefetch"],
         "true" if shape["nontemporal"] else "false")
     del run32
     torch.cuda.empty_cache()
@@ -707,7 +707,7 @@ def main() -> int:
         except Exception as exc:  # the baseline is reported, never required
             log(f"cpu baseline failed: {exc}")
 
-    traffic = read_traffic(ROOT / "profiles" / f"westervelt_march_{n}_pmc.txt")
+    traffic = read_traffic(ROOT / "profiles" / f"westervelt_march_{n}_pmc.txt", kname)
     out = {
         "metric": "Westervelt Mcell-steps/s & %HBM roofline; Gilmore bubble-steps/s; vs FP64 plain",
         "value": round(value, 2),

@@ -76,8 +76,9 @@ struct AdjPlane {
 
 // The adjoint field is stored as w = G lam (zero outside the interior), so
 // the sponge factor is formed once per cell when lam^n is written.
-template <typename T, int kAX, int kAY>
+template <typename T, int kAX, int kAY, int PF>
 __global__ void __launch_bounds__(kAX * kAY) adj_step_kernel(AdjArgs<T> a) {
+    static_assert(PF == 1 || PF == 2, "planes of loads in flight: 1 or 2");
     static_assert(kAY >= 3 && kAX >= 6, "tile too small for the 3-cell halo");
     __shared__ T s[2][kAY + 6][kAX + 6];
     const int lb = xcd_remap((int)blockIdx.x, a.nblocks);
@@ -129,9 +130,13 @@ __global__ void __launch_bounds__(kAX * kAY) adj_step_kernel(AdjArgs<T> a) {
         // source plane iz=2: only the z stencil reaches interior cells
         a.lsrc[(size_t)iy * N + ix] = a.kz * (T(270.0) * q[3] - T(27.0) * q[4] + T(2.0) * q[5]) / T(180.0);
     }
-    AdjPlane<T> cur;
+    // the loads of the next PF planes are in flight while plane z is computed
+    AdjPlane<T> cur, nx1;
     load_plane(z0, cur);
+    if (PF == 2 && z0 + 1 < z1) load_plane(z0 + 1, nx1);
     for (int z = z0; z < z1; ++z) {
+        AdjPlane<T> nxt;
+        if (z + PF < z1) load_plane(z + PF, nxt);
         const int b = (z - z0) & 1;
         q[6] = cur.wq;
         s[b][ty + 3][tx + 3] = q[3];
@@ -173,7 +178,12 @@ __global__ void __launch_bounds__(kAX * kAY) adj_step_kernel(AdjArgs<T> a) {
         }
 #pragma unroll
         for (int k = 0; k < 6; ++k) q[k] = q[k + 1];
-        if (z + 1 < z1) load_plane(z + 1, cur);
+        if (PF == 1) {
+            cur = nxt;
+        } else {
+            cur = nx1;
+            nx1 = nxt;
+        }
     }
 }
 
@@ -414,13 +424,32 @@ static int adjoint_t(const bd_westervelt_params* prm, int Nxy, int Nz, int Nt, i
         BD_HIP_CHECK(hipEventRecord(recomputed[sl], s2));
         return BD_OK;
     };
+    // planes of loads in flight (BD_ADJ_PF = 1 or 2, default 2)
+    int pf = 2;
+    if (const char* v = std::getenv("BD_ADJ_PF")) pf = std::atoi(v) == 1 ? 1 : 2;
     auto launch_adj = [&](hipStream_t s_) -> int {
-        switch (tile) {
-            case 0: hipLaunchKernelGGL((adj_step_kernel<T, 64, 4>), dim3(A.nblocks), dim3(256), 0, s_, A); break;
-            case 1: hipLaunchKernelGGL((adj_step_kernel<T, 64, 8>), dim3(A.nblocks), dim3(512), 0, s_, A); break;
-            case 2: hipLaunchKernelGGL((adj_step_kernel<T, 32, 8>), dim3(A.nblocks), dim3(256), 0, s_, A); break;
-            case 3: hipLaunchKernelGGL((adj_step_kernel<T, 128, 4>), dim3(A.nblocks), dim3(512), 0, s_, A); break;
-            default: hipLaunchKernelGGL((adj_step_kernel<T, 128, 8>), dim3(A.nblocks), dim3(1024), 0, s_, A); break;
+        if (pf == 1) {
+            switch (tile) {
+                case 0: hipLaunchKernelGGL((adj_step_kernel<T, 64, 4, 1>), dim3(A.nblocks), dim3(256), 0, s_, A); break;
+                case 1: hipLaunchKernelGGL((adj_step_kernel<T, 64, 8, 1>), dim3(A.nblocks), dim3(512), 0, s_, A); break;
+                case 2: hipLaunchKernelGGL((adj_step_kernel<T, 32, 8, 1>), dim3(A.nblocks), dim3(256), 0, s_, A); break;
+                case 3: hipLaunchKernelGGL((adj_step_kernel<T, 128, 4, 1>), dim3(A.nblocks), dim3(512), 0, s_, A); break;
+                default: hipLaunchKernelGGL((adj_step_kernel<T, 128, 8, 1>), dim3(A.nblocks), dim3(1024), 0, s_, A); break;
+            }
+        } else {
+            switch (tile) {
+                case 0: hipLaunchKernelGGL((adj_step_kernel<T, 64, 4, 2>), dim3(A.nblocks), dim3(256), 0, s_, A); break;
+                case 1: hipLaunchKernelGGL((adj_step_kernel<T, 64, 8, 2>), dim3(A.nblocks), dim3(512), 0, s_, A); break;
+                case 2: hipLaunchKernelGGL((adj_step_kernel<T, 32, 8, 2>), dim3(A.nblocks), dim3(256), 0, s_, A); break;
+                case 3: hipLaunchKernelGGL((adj_step_kernel<T, 128, 4, 2>), dim3(A.nblocks), dim3(512), 0, s_, A); break;
+                default:
+                    // 1024 threads leave 128 VGPRs: the FP64 two-plane pipeline would spill
+                    if constexpr (sizeof(T) == 8)
+                        hipLaunchKernelGGL((adj_step_kernel<T, 128, 8, 1>), dim3(A.nblocks), dim3(1024), 0, s_, A);
+                    else
+                        hipLaunchKernelGGL((adj_step_kernel<T, 128, 8, 2>), dim3(A.nblocks), dim3(1024), 0, s_, A);
+                    break;
+            }
         }
         BD_LAUNCH_CHECK();
         return BD_OK;

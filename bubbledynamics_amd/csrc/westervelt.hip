@@ -317,8 +317,13 @@ constexpr int kDefaultBlocks = 1024;
 // 224^3 122.6 / 125.7 / 122.7, 256^3 166.0 / 167.0 / 166.0; FP64 192^3
 // variants 0 / 10 / 13: 158.3 / 149.4 / 159.6.  No geometric rule separates
 // them, so the plan measures.
-constexpr int kTuneF32[] = {kDeepVariant, 10, 13};
-constexpr int kTuneF64[] = {kDefaultVariant, 10};
+constexpr int kTuneF32[] = {13, 10, kDeepVariant};
+constexpr int kTuneF64[] = {10, kDefaultVariant};
+// The tile taken without a measurement (plans without prepare, the
+// adjoint's forward recompute): one row per thread with three planes of
+// loads in flight for FP32 (never more than 5% off the fastest above, 16%
+// ahead at 128^3), two for FP64.
+constexpr int kStaticF32 = 13, kStaticF64 = 10;
 
 // The 12-field state does not fit in the last-level cache: the step kernel
 // then loads the history levels with the non-temporal policy.
@@ -370,21 +375,13 @@ static WvGeom make_geom(const bd_westervelt_params* p, int Nxy, int Nz, int vari
     return g;
 }
 
-// Variant / target block count for the interior kernel; BD_WV_VARIANT and
-// BD_WV_BLOCKS override the defaults (used by the tile-shape sweep).  FP32
-// takes the two-row prefetching tile unless its z-chunks (at least 8 planes)
-// leave it fewer than 3/4 of the one-row tile's workgroups, as on small grids;
-// FP64, whose two-row tile holds twice the registers, takes the one-row tile.
-// FP32 steps measured on one MI355X (quarter domain, us, one-row / two-row):
-// 128^3 26.7 / 30.4 (512 vs 1024 workgroups), 160^3 47.6 / 46.7,
-// 192^3 76.3 / 74.0, 208^3 135.8 / 116.3, 224^3 144.0 / 120.9,
-// 256^3 204.0 / 188.0, 320^3 424.3 / 410.2; FP64 192^3 171.3 / 176.3.
+// Variant / target block count for the interior kernel without a
+// measurement (kStaticF32 / kStaticF64); BD_WV_VARIANT and BD_WV_BLOCKS
+// override them (used by the tile-shape sweep).
 static void launch_shape(const bd_westervelt_params* p, int Nxy, int Nz, bool f64, int* variant, int* blocks) {
-    *variant = kDefaultVariant;
+    (void)p; (void)Nxy; (void)Nz;
+    *variant = f64 ? kStaticF64 : kStaticF32;
     *blocks = kDefaultBlocks;
-    if (!f64 && 4 * make_geom(p, Nxy, Nz, kDeepVariant, kDefaultBlocks).nblocks >=
-                    3 * make_geom(p, Nxy, Nz, kDefaultVariant, kDefaultBlocks).nblocks)
-        *variant = kDeepVariant;
     if (const char* v = std::getenv("BD_WV_VARIANT")) {
         const int x = std::atoi(v);
         if (x >= 0 && x < kWvNumVariants) *variant = x;
