@@ -275,10 +275,18 @@ def coupled_figures() -> dict:
     cells = (n - 2) ** 3
     rate = cells * nsub / secs
     T = run.temperature()
+    # substeps run in pairs: a pair reads T, Q, TD, TD_comp and writes T'', TD,
+    # TD_comp once (7 FP32 words per cell), 14 B per cell-substep; the
+    # measured traffic per pair is in the committed counter summary
+    traffic = read_traffic(ROOT / "profiles" / "bioheat_pair_256_trace.txt", "bh_march2")
+    pair_cells = 2 * cells
     return {"westervelt_steps": tq["steps"], "westervelt_s": tq["settle_s"] + tq["window_s"],
             "q_max_w_m3": float(Q.max().item()), "bioheat_substeps": nsub, "bioheat_s": secs,
             "bioheat_mcell_steps_per_s": rate / 1e6,
-            "bioheat_hbm_frac_28B": rate * 28 / PEAK_HBM,
+            "bioheat_alg_bytes_per_cell_substep": 14,
+            "bioheat_hbm_frac_14B": rate * 14 / PEAK_HBM,
+            "bioheat_actual_gb_s": None if traffic is None else rate / pair_cells * traffic / 1e9,
+            "bioheat_single_substep_equiv_gb_s_28B": rate * 28 / 1e9,
             "peak_rise_k": float((T.max() - run.p.T_art).item()),
             "max_cem43_min": float(run.dose().max().item() / 60.0)}
 

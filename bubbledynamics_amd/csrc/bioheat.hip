@@ -36,34 +36,62 @@ struct BhArgs {
 
 constexpr int kBX = 64, kBY = 4;
 
-// One cell of one substep (T:83-97): conduction, source, Pennes perfusion.
+// Coefficients of one medium for one substep, selected once per plane.
+// Perfusion acts in tissue only; elsewhere its coefficient is zero, and
+// Tn - 0 * (Tc - theta) is Tn for every finite temperature.
+template <typename T>
+struct BhCoef { T keep, kxy, kz, dq, perf, theta; };
+
+template <typename T>
+__device__ __forceinline__ BhCoef<T> bh_coef(const BhArgs<T>& a, int reg) {
+    return {a.keep[reg], a.kxy[reg], a.kz[reg], a.dq[reg], reg ? a.perf : T(0), a.theta_art};
+}
+
+// One cell of one substep (T:83-97) in the library's order: conduction,
+// source, Pennes perfusion.
+template <typename T>
+__device__ __forceinline__ T bh_update_c(const BhCoef<T>& c, T Tc, T l, T r, T d, T u, T zp, T zm, T q) {
+    const T Tn = Tc * c.keep + c.kxy * ((l + r) + (u + d)) + c.kz * (zp + zm) + c.dq * q;
+    return Tn - c.perf * (Tc - c.theta);
+}
+
+// One cell of one substep: the reference's expression order (T:85-96) for
+// the FP64 drop-in, else bh_update_c.
 template <typename T>
 __device__ __forceinline__ T bh_update(const BhArgs<T>& a, int reg, T Tc, T l, T r, T d, T u, T zp, T zm, T q) {
-    T Tn;
-    if (a.reference_order) {
-        Tn = Tc * (T(1.0) - T(4.0) * a.kxy[reg] - T(2.0) * a.kz[reg]) + a.kxy[reg] * (l + r + u + d) +
-             a.kz[reg] * (zp + zm) + a.dq[reg] * q;
-    } else {
-        Tn = Tc * a.keep[reg] + a.kxy[reg] * ((l + r) + (u + d)) + a.kz[reg] * (zp + zm) + a.dq[reg] * q;
-    }
+    if (!a.reference_order) return bh_update_c(bh_coef(a, reg), Tc, l, r, d, u, zp, zm, q);
+    T Tn = Tc * (T(1.0) - T(4.0) * a.kxy[reg] - T(2.0) * a.kz[reg]) + a.kxy[reg] * (l + r + u + d) +
+           a.kz[reg] * (zp + zm) + a.dq[reg] * q;
     if (reg && a.perf != T(0)) Tn = Tn - a.perf * (Tc - a.theta_art);
     return Tn;
 }
 
+// CEM43 dose constants: stored value + T_off = T [K]; dose_off = (T_off -
+// 273.15) - 43 for the FP32 exponent.
+template <typename T>
+struct BhDoseC { T dose_off; double T_off, dt; };
+
 // CEM43 dose increment of one substep at the new temperature (T:99-102).
 template <typename T>
-__device__ __forceinline__ T bh_dose(const BhArgs<T>& a, T Tn) {
+__device__ __forceinline__ T bh_dose_c(const BhDoseC<T>& c, T Tn) {
     if constexpr (std::is_same<T, float>::value) {
-        const float x = Tn + a.dose_off;               // T_c - 43
-        return exp2f(x < 0.0f ? 2.0f * x : x) * (float)a.dt_T;
+        // T_c - 43; the hardware exp2 equals exp2f wherever the result is a
+        // normal number (T_c above -20 C)
+        const float x = Tn + c.dose_off;
+        return __builtin_amdgcn_exp2f(x < 0.0f ? 2.0f * x : x) * (float)c.dt;
     } else {
         // T:99-102 writes pow(4, T_c - 43) and pow(2, T_c - 43); they are
         // evaluated as exp2(2 (T_c - 43)) and exp2(T_c - 43), the same values
         // (2x is exact) up to the last bit of the library function, at a
         // fraction of pow's cost (FP64 pair kernel: 66 -> 149 G cell-substeps/s)
-        const double T_c = (Tn + a.T_off) - 273.15;   // T:88, T:96
-        return (T_c < 43.0) ? exp2(2.0 * (T_c - 43.0)) * a.dt_T : exp2(T_c - 43.0) * a.dt_T;
+        const double T_c = (Tn + c.T_off) - 273.15;   // T:88, T:96
+        return (T_c < 43.0) ? exp2(2.0 * (T_c - 43.0)) * c.dt : exp2(T_c - 43.0) * c.dt;
     }
+}
+
+template <typename T>
+__device__ __forceinline__ T bh_dose(const BhArgs<T>& a, T Tn) {
+    return bh_dose_c(BhDoseC<T>{a.dose_off, a.T_off, a.dt_T}, Tn);
 }
 
 // TD += add, Kahan-compensated when the compensation field is in use.
@@ -159,18 +187,27 @@ __global__ void __launch_bounds__(kBX * kBY) bh_march(BhArgs<T> a) {
 // z: the workgroup owns output planes [z0, z1) and runs substep 1 on planes
 // z0-1 .. z1.  Substep-1 values live in registers (own column, 3 planes) and
 // in an LDS plane (x/y neighbours); the dose of both substeps is added before
-// TD is stored once.  The per-cell arithmetic is bh_update / bh_dose /
+// TD is stored once.  The per-cell arithmetic is bh_update_c / bh_dose_c /
 // bh_accumulate in the order of two single substeps, so the result is
 // bit-identical to them.  Substep-1 values of grid-boundary cells (never
 // updated) are read from `dst`, which the first of two single substeps
 // would have left there; the result goes to `dst`.
-template <typename T, int BX, int TY, int RY>
+// Few instructions per cell: loads and stores of cells a lane does not own
+// take an out-of-range buffer offset (loads return 0, stores are dropped),
+// except the boundary and halo loads, which a wavefront without such a lane
+// skips; every lane evaluates both updates and keeps what it owns; both
+// media's coefficients come from LDS; the compensated dose is a template
+// choice.  Every load of plane zs+1 is issued before plane zs is computed.
+template <typename T, int BX, int TY, int RY, bool KAHAN>
 __global__ void __launch_bounds__(BX * TY) bh_march2(BhArgs<T> a) {
     constexpr int ROWS = TY * RY;
-    // the y-halo rows below and above the region come from the first and last thread row
-    static_assert(TY >= 2 && RY >= 1, "two thread rows at least");
+    // the y-halo rows below and above the region come from the first and last
+    // thread row, and a wavefront holds one thread row (uniform halo branches)
+    static_assert(TY >= 2 && RY >= 1 && BX % 64 == 0, "two thread rows of whole wavefronts");
+    constexpr uint32_t OOB = 0xFFFFFFFFu;
+    constexpr bool kahan = KAHAN;
     __shared__ T sa[2][ROWS + 2][BX + 2];    // T^n plane zs with a one-cell halo
-    __shared__ T sb[2][ROWS][BX];            // substep-1 plane
+    __shared__ T sb[2][ROWS + 2][BX + 2];    // substep-1 plane (its ring is never read into a kept value)
     const int lb = xcd_remap((int)blockIdx.x, a.nblocks);
     const int zc = lb / (a.ntx * a.nty), t2 = lb % (a.ntx * a.nty);   // tile-major
     const int bx = t2 % a.ntx, by = t2 / a.ntx;
@@ -183,128 +220,161 @@ __global__ void __launch_bounds__(BX * TY) bh_march2(BhArgs<T> a) {
     const int gy0 = by * (ROWS - 2);
     const int z0 = 1 + zc * a.zchunk;
     const int z1 = min(z0 + a.zchunk, a.Nz - 1);
-    const bool kahan = a.TDc != nullptr;
+    const BhDoseC<T> dc{a.dose_off, a.T_off, a.dt_T};
+    // both media's coefficients in LDS: a plane's medium is read there (one
+    // broadcast read) instead of being selected into registers on every lane
+    __shared__ T scoef[2][6];
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const BhCoef<T> c = bh_coef(a, m);
+            scoef[m][0] = c.keep; scoef[m][1] = c.kxy; scoef[m][2] = c.kz;
+            scoef[m][3] = c.dq; scoef[m][4] = c.perf; scoef[m][5] = c.theta;
+        }
+    }
+    auto coef = [&](int m) {
+        return BhCoef<T>{scoef[m][0], scoef[m][1], scoef[m][2], scoef[m][3], scoef[m][4], scoef[m][5]};
+    };
 
-    int gy[RY];
-    bool ing[RY], itr[RY], outc[RY];
-    uint32_t o[RY], cq[RY];
+    // per row: offsets of the cell as a grid cell (o_in), as a grid cell that
+    // substep 1 leaves unchanged (o_edge), as an output cell (o_out), of the
+    // shifted source (cq) and of the x halo (ohx); OOB where the row's cell
+    // is not one
+    bool itr[RY];
+    uint32_t o_in[RY], o_edge[RY], o_out[RY], cq[RY], ohx[RY];
+    const int hxd = tx == 0 ? -1 : (tx == BX - 1 ? 1 : 0);
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
         const int ly = ty * RY + r;
-        gy[r] = gy0 + ly;
-        ing[r] = gx < N && gy[r] < N;
-        itr[r] = gx >= 1 && gx <= N - 2 && gy[r] >= 1 && gy[r] <= N - 2;
-        outc[r] = itr[r] && tx >= 1 && tx <= BX - 2 && ly >= 1 && ly <= ROWS - 2;
-        o[r] = ing[r] ? ((uint32_t)gy[r] * (uint32_t)N + (uint32_t)gx) * ES : 0u;
-        const int ixa = gx - a.shift_x, iya = gy[r] - a.shift_y;
+        const int gy = gy0 + ly;
+        const bool ing = gx < N && gy < N;
+        itr[r] = gx >= 1 && gx <= N - 2 && gy >= 1 && gy <= N - 2;
+        const bool outc = itr[r] && tx >= 1 && tx <= BX - 2 && ly >= 1 && ly <= ROWS - 2;
+        const uint32_t o = ((uint32_t)gy * (uint32_t)N + (uint32_t)gx) * ES;
+        o_in[r] = ing ? o : OOB;
+        o_edge[r] = ing && !itr[r] ? o : OOB;
+        o_out[r] = outc ? o : OOB;
+        const int ixa = gx - a.shift_x, iya = gy - a.shift_y;
         const bool q_in = itr[r] && ixa >= 0 && ixa < N && iya >= 0 && iya < N;   // T:76
-        cq[r] = q_in ? ((uint32_t)iya * (uint32_t)N + (uint32_t)ixa) * ES : 0xFFFFFFFFu;
-    }
-    // halo of the region: column -1 / 64 (edge lanes, every row), row -1 / ROWS
-    const int hxd = tx == 0 ? -1 : (tx == BX - 1 ? 1 : 0);
-    uint32_t ohx[RY];
-#pragma unroll
-    for (int r = 0; r < RY; ++r) {
+        cq[r] = q_in ? ((uint32_t)iya * (uint32_t)N + (uint32_t)ixa) * ES : OOB;
         const int hx = gx + hxd;
-        ohx[r] = (hxd != 0 && gy[r] < N && hx >= 0 && hx < N) ? ((uint32_t)gy[r] * (uint32_t)N + (uint32_t)hx) * ES
-                                                              : 0xFFFFFFFFu;
+        ohx[r] = (hxd != 0 && gy < N && hx >= 0 && hx < N) ? ((uint32_t)gy * (uint32_t)N + (uint32_t)hx) * ES : OOB;
     }
+    // halo rows of the region: row -1 (thread row 0) and row ROWS (last thread row)
     const int hyd = ty == 0 ? -1 : (ty == TY - 1 ? 1 : 0);
     const int hyrow = hyd < 0 ? gy0 - 1 : gy0 + ROWS;
     const uint32_t ohy = (hyd != 0 && gx < N && hyrow >= 0 && hyrow < N)
-                             ? ((uint32_t)hyrow * (uint32_t)N + (uint32_t)gx) * ES : 0xFFFFFFFFu;
+                             ? ((uint32_t)hyrow * (uint32_t)N + (uint32_t)gx) * ES : OOB;
 
     struct P2 { T zp[RY], q[RY], td[RY], tdc[RY], bnd[RY], hx[RY], hy; };
-    // loads of step-1 plane zs (zs in [0, Nz-1]); the plane above the last is not read
+    // loads of step-1 plane zs (zs in [0, Nz-1]); zp is plane zs+1 (0 past the last plane)
     auto load = [&](int zs, P2& v) {
         const size_t zo = (size_t)zs * plane;
-        const rsrc_t rs = make_rsrc(a.src + zo, (size_t)min(2, a.Nz - zs) * pbytes);
+        const rsrc_t rs = make_rsrc(a.src + zo, pbytes);
+        const rsrc_t rn = make_rsrc(a.src + zo + plane, zs + 1 < a.Nz ? pbytes : 0u);
         const rsrc_t rq = make_rsrc(a.Q + zo, pbytes);
         const rsrc_t rb = make_rsrc(a.dst + zo, pbytes);
+        const rsrc_t rd = make_rsrc(a.TD + zo, pbytes);
         const bool bplane = zs == 0 || zs == a.Nz - 1;
         const bool outp = zs >= z0 && zs < z1;
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            v.zp[r] = (ing[r] && zs + 1 < a.Nz) ? bload<T>(rs, o[r], pbytes) : T(0);
+            const uint32_t od = outp ? o_out[r] : OOB;
+            const uint32_t ob = bplane ? o_in[r] : o_edge[r];
+            v.zp[r] = bload<T>(rn, o_in[r]);
             v.q[r] = bload<T>(rq, cq[r]);
-            v.bnd[r] = (ing[r] && (bplane || !itr[r])) ? bload<T>(rb, o[r]) : T(0);
-            v.td[r] = (outc[r] && outp) ? bload<T>(make_rsrc(a.TD + zo, pbytes), o[r]) : T(0);
-            v.tdc[r] = (outc[r] && outp && kahan) ? bload<T>(make_rsrc(a.TDc + zo, pbytes), o[r]) : T(0);
-            v.hx[r] = bload<T>(rs, ohx[r]);
+            // boundary and halo loads concern few lanes: a wavefront without
+            // such a lane skips them
+            v.bnd[r] = ob != OOB ? bload<T>(rb, ob) : T(0);
+            v.td[r] = bload<T>(rd, od);
+            v.hx[r] = ohx[r] != OOB ? bload<T>(rs, ohx[r]) : T(0);
         }
-        v.hy = bload<T>(rs, ohy);
+        if constexpr (KAHAN) {
+            const rsrc_t rc = make_rsrc(a.TDc + zo, pbytes);
+#pragma unroll
+            for (int r = 0; r < RY; ++r) v.tdc[r] = bload<T>(rc, outp ? o_out[r] : OOB);
+        } else {
+#pragma unroll
+            for (int r = 0; r < RY; ++r) v.tdc[r] = T(0);
+        }
+        v.hy = ohy != OOB ? bload<T>(rs, ohy) : T(0);
     };
 
     const int zb = z0 - 1, ze = z1;           // step-1 planes
     T tm[RY], tc[RY], t1m[RY], t1c[RY], qp[RY], hd[RY], hdc[RY];
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
-        tm[r] = (ing[r] && zb >= 1) ? a.src[(size_t)(zb - 1) * plane + o[r] / ES] : T(0);
-        tc[r] = ing[r] ? a.src[(size_t)zb * plane + o[r] / ES] : T(0);
+        const rsrc_t rs = make_rsrc(a.src + (size_t)zb * plane, pbytes);
+        const rsrc_t rm = make_rsrc(a.src + (size_t)(zb > 0 ? zb - 1 : 0) * plane, zb >= 1 ? pbytes : 0u);
+        tm[r] = bload<T>(rm, o_in[r]);
+        tc[r] = bload<T>(rs, o_in[r]);
         t1m[r] = t1c[r] = qp[r] = hd[r] = hdc[r] = T(0);
     }
-    P2 cur, nxt;
-    load(zb, cur);
-    for (int zs = zb; zs <= ze; ++zs) {
-        if (zs < ze) load(zs + 1, nxt);
-        const int i = zs - zb, b = i & 1;
+
+    // one plane: cur holds the loads of plane zs; the loads of plane zs+1 go to nxt
+    auto body = [&](int zs, P2& cur, P2& nxt) {
+        if (zs + 1 <= ze) load(zs + 1, nxt);
+        const int b = (zs - zb) & 1;
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            const int ly = ty * RY + r + 1;
-            sa[b][ly][tx + 1] = tc[r];
-            if (hxd) sa[b][ly][tx + 1 + hxd] = cur.hx[r];
+            const int lp = ty * RY + r + 1;
+            sa[b][lp][tx + 1 + hxd] = cur.hx[r];       // halo column on edge lanes (own cell otherwise,
+            sa[b][lp][tx + 1] = tc[r];                 // overwritten by this lane's next write)
         }
         if (hyd < 0) sa[b][0][tx + 1] = cur.hy;
         if (hyd > 0) sa[b][ROWS + 1][tx + 1] = cur.hy;
         __syncthreads();
         // substep 1 at plane zs on the region
-        const bool bplane = zs == 0 || zs == a.Nz - 1;
-        const int reg1 = zs >= a.iz_tissue ? 1 : 0;
-        const bool outp = zs >= z0 && zs < z1;
+        const bool upd1 = !(zs == 0 || zs == a.Nz - 1);
+        const BhCoef<T> c1 = coef(zs >= a.iz_tissue ? 1 : 0);      // after the barrier above
         T t1[RY];
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            const int ly = ty * RY + r + 1;
-            if (!ing[r]) {
-                t1[r] = T(0);
-            } else if (bplane || !itr[r]) {
-                t1[r] = cur.bnd[r];
-            } else {
-                t1[r] = bh_update(a, reg1, tc[r], sa[b][ly][tx], sa[b][ly][tx + 2], sa[b][ly - 1][tx + 1],
-                                  sa[b][ly + 1][tx + 1], cur.zp[r], tm[r], cur.q[r]);
-            }
-            sb[b][ly - 1][tx] = t1[r];
+            const int lp = ty * RY + r + 1;
+            const T u = bh_update_c(c1, tc[r], sa[b][lp][tx], sa[b][lp][tx + 2], sa[b][lp - 1][tx + 1],
+                                    sa[b][lp + 1][tx + 1], cur.zp[r], tm[r], cur.q[r]);
+            t1[r] = (upd1 && itr[r]) ? u : cur.bnd[r];
+            sb[b][lp][tx + 1] = t1[r];
         }
         // substep 2 at plane zs - 1 on the inner cells (substep-1 plane zs-1
         // was written to sb[b^1] before this iteration's barrier)
         const int zo2 = zs - 1;
         if (zo2 >= z0) {
-            const int reg2 = zo2 >= a.iz_tissue ? 1 : 0;
+            const BhCoef<T> c2 = coef(zo2 >= a.iz_tissue ? 1 : 0);
             const size_t zo = (size_t)zo2 * plane;
             const int pb = b ^ 1;
+            const rsrc_t rdst = make_rsrc(a.dst + zo, pbytes);
+            const rsrc_t rd = make_rsrc(a.TD + zo, pbytes);
+            const rsrc_t rc = make_rsrc(KAHAN ? a.TDc + zo : a.TD + zo, KAHAN ? pbytes : 0u);
 #pragma unroll
             for (int r = 0; r < RY; ++r) {
-                if (!outc[r]) continue;
-                const int ly = ty * RY + r;
-                const T T2 = bh_update(a, reg2, t1c[r], sb[pb][ly][tx - 1], sb[pb][ly][tx + 1],
-                                       sb[pb][ly - 1][tx], sb[pb][ly + 1][tx], t1[r], t1m[r], qp[r]);
-                bstore<T>(T2, make_rsrc(a.dst + zo, pbytes), o[r]);
+                const int lp = ty * RY + r + 1;
+                const T T2 = bh_update_c(c2, t1c[r], sb[pb][lp][tx], sb[pb][lp][tx + 2], sb[pb][lp - 1][tx + 1],
+                                         sb[pb][lp + 1][tx + 1], t1[r], t1m[r], qp[r]);
                 T td = hd[r], tdc = hdc[r];
-                bh_accumulate(td, tdc, bh_dose(a, T2), kahan);
-                if (kahan) bstore<T>(tdc, make_rsrc(a.TDc + zo, pbytes), o[r]);
-                bstore<T>(td, make_rsrc(a.TD + zo, pbytes), o[r]);
+                bh_accumulate(td, tdc, bh_dose_c(dc, T2), kahan);
+                bstore<T>(T2, rdst, o_out[r]);
+                if constexpr (KAHAN) bstore<T>(tdc, rc, o_out[r]);
+                bstore<T>(td, rd, o_out[r]);
             }
         }
         // dose of substep 1 at plane zs, held until substep 2 reaches it
+        // (lanes and planes without an output cell hold a value never stored)
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
             T td = cur.td[r], tdc = cur.tdc[r];
-            if (outc[r] && outp) bh_accumulate(td, tdc, bh_dose(a, t1[r]), kahan);
+            bh_accumulate(td, tdc, bh_dose_c(dc, t1[r]), kahan);
             hd[r] = td; hdc[r] = tdc;
             t1m[r] = t1c[r]; t1c[r] = t1[r];
             qp[r] = cur.q[r];
             tm[r] = tc[r]; tc[r] = cur.zp[r];
         }
+    };
+
+    P2 cur, nxt;
+    load(zb, cur);
+    for (int zs = zb; zs <= ze; ++zs) {
+        body(zs, cur, nxt);
         cur = nxt;
     }
 }
@@ -351,45 +421,43 @@ static int launch_bh(const bd_bioheat_params* p, const T* src, T* dst, const T* 
     return BD_OK;
 }
 
-// Two fused substeps (bh_march2): FP32 regions of 64 x 16 (4 thread rows of
-// 4 rows), FP64 64 x 8.
-template <typename T>
-static int launch_bh2(const bd_bioheat_params* p, const T* src, T* dst, const T* Q, T* TD, T* TDc, int Nxy,
-                      int Nz, int sx, int sy, hipStream_t st) {
-    constexpr int BX = 64;
-    constexpr int TY = 4;
-    constexpr int RY = sizeof(T) == 4 ? 4 : 2;
+// Region shapes of the pair kernel: BX columns (BX - 2 output columns), TY
+// thread rows of RY rows each (TY * RY - 2 output rows).
+struct Bh2Variant { int bx, ty, ry; };
+constexpr Bh2Variant kBh2Variants[] = {{64, 4, 4}, {128, 4, 4}, {256, 4, 4}, {64, 4, 2}, {128, 4, 2}, {256, 4, 2}};
+constexpr int kBh2NumVariants = (int)(sizeof(kBh2Variants) / sizeof(kBh2Variants[0]));
+// FP64 keeps every value in registers with two rows per thread
+constexpr bool bh2_f64_ok(int v) { return kBh2Variants[v].ry <= 2; }
+
+template <typename T, int BX, int TY, int RY, bool KAHAN>
+static int run_bh2(BhArgs<T> a, int variant, hipStream_t st) {
     constexpr int ROWS = TY * RY;
-    BhArgs<T> a;
-    a.src = src; a.dst = dst; a.Q = Q; a.TD = TD; a.TDc = TDc;
-    a.Nxy = Nxy; a.Nz = Nz;
-    a.iz_tissue = first_tissue_plane_bh(p, Nz);
-    a.shift_x = sx; a.shift_y = sy;
-    const int nin = Nxy - 2;
+    const int nin = a.Nxy - 2;
     a.ntx = (nin + (BX - 2) - 1) / (BX - 2);
     a.nty = (nin + (ROWS - 2) - 1) / (ROWS - 2);
-    const int nzin = Nz - 2;
+    const int nzin = a.Nz - 2;
     // z-chunk length: each workgroup runs zchunk + 2 substep-1 planes, and the
     // launch runs in ceil(workgroups / resident) rounds; take the length of
     // least rounds * (zchunk + 2), at least 8 planes (BD_BH_ZCHUNK overrides)
     constexpr int kMaxDev = 64;
-    static int resident[kMaxDev][2] = {};
+    static int resident[kMaxDev][2][2][kBh2NumVariants] = {};
     static std::mutex mtx;
     int dev = 0, res = 0;
     BD_HIP_CHECK(hipGetDevice(&dev));
     if (dev < 0 || dev >= kMaxDev) return BD_ERR_INVALID;
     {
         std::lock_guard<std::mutex> lock(mtx);
-        res = resident[dev][sizeof(T) == 8];
+        res = resident[dev][sizeof(T) == 8][KAHAN][variant];
     }
     if (res == 0) {
         int per_cu = 0;
         hipDeviceProp_t prop;
         BD_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
-        BD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bh_march2<T, BX, TY, RY>, BX * TY, 0));
+        BD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bh_march2<T, BX, TY, RY, KAHAN>,
+                                                                  BX * TY, 0));
         res = std::max(1, per_cu) * prop.multiProcessorCount;
         std::lock_guard<std::mutex> lock(mtx);
-        resident[dev][sizeof(T) == 8] = res;
+        resident[dev][sizeof(T) == 8][KAHAN][variant] = res;
     }
     const long long tiles = (long long)a.ntx * a.nty;
     int zchunk = nzin;
@@ -405,6 +473,45 @@ static int launch_bh2(const bd_bioheat_params* p, const T* src, T* dst, const T*
     a.nzc = (nzin + zchunk - 1) / zchunk;
     a.zchunk = zchunk;
     a.nblocks = (int)(tiles * a.nzc);
+    hipLaunchKernelGGL((bh_march2<T, BX, TY, RY, KAHAN>), dim3(a.nblocks), dim3(BX * TY), 0, st, a);
+    BD_LAUNCH_CHECK();
+    return BD_OK;
+}
+
+template <typename T, int BX, int TY, int RY>
+static int run_bh2_k(const BhArgs<T>& a, int variant, hipStream_t st) {
+    return a.TDc ? run_bh2<T, BX, TY, RY, true>(a, variant, st) : run_bh2<T, BX, TY, RY, false>(a, variant, st);
+}
+
+// The pair kernel's region for a grid: rows of four (FP32) or two (FP64)
+// per thread, and the width of 64, 128 or 256 columns that leaves the fewest
+// idle lanes across the interior (the widest on a tie: fewer overlapping
+// columns).  BD_BH_VARIANT overrides it, for the region sweep.
+template <typename T>
+static int bh2_variant(int Nxy) {
+    const int base = sizeof(T) == 8 ? 3 : 0;
+    const int nin = Nxy - 2;
+    int v = base, best = -1;
+    for (int k = 0; k < 3; ++k) {
+        const int bx = kBh2Variants[base + k].bx;
+        const int idle = (nin + bx - 3) / (bx - 2) * bx - nin;
+        if (best < 0 || idle <= best) { best = idle; v = base + k; }
+    }
+    if (const char* e = std::getenv("BD_BH_VARIANT")) {
+        const int x = std::atoi(e);
+        if (x >= 0 && x < kBh2NumVariants && (sizeof(T) == 4 || bh2_f64_ok(x))) v = x;
+    }
+    return v;
+}
+
+template <typename T>
+static int launch_bh2(const bd_bioheat_params* p, const T* src, T* dst, const T* Q, T* TD, T* TDc, int Nxy,
+                      int Nz, int sx, int sy, hipStream_t st) {
+    BhArgs<T> a;
+    a.src = src; a.dst = dst; a.Q = Q; a.TD = TD; a.TDc = TDc;
+    a.Nxy = Nxy; a.Nz = Nz;
+    a.iz_tissue = first_tissue_plane_bh(p, Nz);
+    a.shift_x = sx; a.shift_y = sy;
     a.kxy[0] = (T)p->kdt_dxy_water; a.kxy[1] = (T)p->kdt_dxy;
     a.kz[0] = (T)p->kdt_dz_water; a.kz[1] = (T)p->kdt_dz;
     a.dq[0] = (T)p->dt_rho_ct_water; a.dq[1] = (T)p->dt_rho_ct;
@@ -416,9 +523,47 @@ static int launch_bh2(const bd_bioheat_params* p, const T* src, T* dst, const T*
     a.dt_T = p->dt_T;
     a.dose_off = (T)((p->T_off - 273.15) - 43.0);
     a.reference_order = 0;
-    hipLaunchKernelGGL((bh_march2<T, BX, TY, RY>), dim3(a.nblocks), dim3(BX * TY), 0, st, a);
-    BD_LAUNCH_CHECK();
-    return BD_OK;
+    const int v = bh2_variant<T>(Nxy);
+    if constexpr (sizeof(T) == 4) {
+        switch (v) {
+            case 0: return run_bh2_k<T, 64, 4, 4>(a, v, st);
+            case 1: return run_bh2_k<T, 128, 4, 4>(a, v, st);
+            case 2: return run_bh2_k<T, 256, 4, 4>(a, v, st);
+            case 3: return run_bh2_k<T, 64, 4, 2>(a, v, st);
+            case 4: return run_bh2_k<T, 128, 4, 2>(a, v, st);
+            default: return run_bh2_k<T, 256, 4, 2>(a, v, st);
+        }
+    } else {
+        switch (v) {
+            case 4: return run_bh2_k<T, 128, 4, 2>(a, v, st);
+            case 5: return run_bh2_k<T, 256, 4, 2>(a, v, st);
+            default: return run_bh2_k<T, 64, 4, 2>(a, v, st);
+        }
+    }
+}
+
+// dst's boundary shell (z planes 0 and Nz-1, rows 0 and N-1, columns 0 and
+// N-1: the cells the update never changes) <- src's.  One thread per shell
+// cell: two z planes, then two y rows and two x columns of each inner plane.
+template <typename T>
+__global__ void bh_shell_copy(const T* __restrict__ src, T* __restrict__ dst, int N, int Nz) {
+    const long long zf = 2LL * N * N, per = 2LL * N + 2LL * (N - 2);
+    const long long total = zf + per * (Nz - 2);
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    long long c;
+    if (i < zf) {
+        const long long k = i % ((long long)N * N);
+        c = (i < (long long)N * N ? 0 : (long long)(Nz - 1) * N * N) + k;
+    } else {
+        const long long j = i - zf;
+        const int iz = 1 + (int)(j / per), t = (int)(j % per);
+        int ix, iy;
+        if (t < 2 * N) { ix = t % N; iy = t < N ? 0 : N - 1; }
+        else { const int u = t - 2 * N; iy = 1 + u % (N - 2); ix = u < N - 2 ? 0 : N - 1; }
+        c = ((long long)iz * N + iy) * N + ix;
+    }
+    dst[c] = src[c];
 }
 
 static int check_bh(const bd_bioheat_params* p, int dtype, int Nxy, int Nz) {
@@ -461,6 +606,20 @@ extern "C" int bd_bioheat_run(const bd_bioheat_params* prm, int dtype, void* T_a
         return BD_ERR_INVALID;
     hipStream_t st = as_stream(stream);
     void* f[2] = {T_a, T_b};
+    if (n_steps > 0) {
+        // the other field takes the current field's boundary shell, so that
+        // substeps paired or not see the same fixed boundary
+        const long long shell = 2LL * Nxy * Nxy + (4LL * Nxy - 4) * (Nz - 2);
+        const unsigned grid = (unsigned)((shell + 255) / 256);
+        const int w = *which;
+        if (dtype == BD_F32)
+            hipLaunchKernelGGL(bh_shell_copy<float>, dim3(grid), dim3(256), 0, st, (const float*)f[w],
+                               (float*)f[1 - w], Nxy, Nz);
+        else
+            hipLaunchKernelGGL(bh_shell_copy<double>, dim3(grid), dim3(256), 0, st, (const double*)f[w],
+                               (double*)f[1 - w], Nxy, Nz);
+        BD_LAUNCH_CHECK();
+    }
     // pairs of substeps in one launch (BD_BH_FUSE=0 runs them one at a time).
     // 256^3, 1,960 substeps (G cell-substeps/s, single / paired):
     // FP32 + Kahan 153 / 222, FP64 87 / 149.

@@ -239,10 +239,11 @@ int bd_bioheat_step(const bd_bioheat_params* prm, int dtype, void* T_out, const 
 /* n_steps substeps of bd_bioheat_step at one source shift, alternating
  * between the fields T_a and T_b: *which (0 or 1) names the field holding the
  * current temperature on entry and on return.  Asynchronous on the stream.
- * Pairs of substeps run in one launch (bit-identical results; each pair
- * flips *which once, so the other field then holds the level two substeps
- * back, and both fields must hold the same boundary values, as the
- * reference's initialisation gives, T:233-257). */
+ * The boundary cells are held fixed: on entry the other field's boundary
+ * shell is set to the current field's (the reference initialises both fields
+ * alike, T:233-257).  Pairs of substeps run in one launch (bit-identical
+ * results; each pair flips *which once, so the other field then holds the
+ * level two substeps back). */
 int bd_bioheat_run(const bd_bioheat_params* prm, int dtype, void* T_a, void* T_b, const void* Q,
                    void* TD, void* TD_comp, int Nxy, int Nz, int shift_x, int shift_y, int n_steps,
                    int* which, void* stream);

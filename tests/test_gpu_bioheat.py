@@ -125,6 +125,64 @@ def test_fused_substep_pairs_bitwise_equal(monkeypatch, dtype, kahan, offset, n,
         assert torch.equal(a.TDc, b.TDc)
 
 
+@pytest.mark.parametrize("dtype,kahan,variant", [(torch.float32, True, v) for v in range(6)] +
+                         [(torch.float32, False, 2), (torch.float64, True, 3), (torch.float64, True, 4),
+                          (torch.float64, True, 5), (torch.float64, False, 5)])
+def test_every_pair_region_bitwise_equal(monkeypatch, dtype, kahan, variant):
+    # each region shape of the pair kernel (64 / 128 / 256 columns, four or
+    # two rows per thread) against single substeps: a grid that ends inside a
+    # region in x and y, two media, perfusion, shifted source
+    n, nz = 131, 29
+    p = two_layer(n, nz)
+    p.T_art = 310.15
+    p.perf = 2e-3
+    a = _random_state(p, n, nz, dtype, kahan, kahan, 5)
+    b = _random_state(p, n, nz, dtype, kahan, kahan, 5)
+    for sx, sy, m in ((2, -3, 6), (-9, 4, 3)):
+        monkeypatch.setenv("BD_BH_VARIANT", str(variant))
+        monkeypatch.delenv("BD_BH_FUSE", raising=False)
+        a.step(sx, sy, m)
+        monkeypatch.delenv("BD_BH_VARIANT", raising=False)
+        monkeypatch.setenv("BD_BH_FUSE", "0")
+        b.step(sx, sy, m)
+    monkeypatch.delenv("BD_BH_FUSE", raising=False)
+    torch.cuda.synchronize()
+    assert torch.equal(a.T[a.cur], b.T[b.cur])
+    assert torch.equal(a.TD, b.TD)
+    if kahan:
+        assert torch.equal(a.TDc, b.TDc)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_differing_boundary_shells_paired_equals_single(monkeypatch, dtype):
+    # the two fields start with different boundary shells: bd_bioheat_run
+    # holds the current field's shell fixed, so paired and single substeps
+    # give the same temperatures (and the shell of the result is the
+    # current field's)
+    n, nz = 70, 41
+    p = two_layer(n, nz)
+    p.T_art = 310.15
+    p.perf = 2e-3
+    runs = []
+    for fuse in ("1", "0"):
+        r = _random_state(p, n, nz, dtype, True, True, 9)
+        r.T[1].add_(dev(np.random.default_rng(2).uniform(-3.0, 3.0, (nz, n, n)), dtype))
+        monkeypatch.setenv("BD_BH_FUSE", fuse)
+        for sx, sy, m in ((1, 2, 5), (0, -4, 4)):
+            r.step(sx, sy, m)
+        runs.append(r)
+    monkeypatch.delenv("BD_BH_FUSE", raising=False)
+    torch.cuda.synchronize()
+    a, b = runs
+    ta, tb = a.T[a.cur], b.T[b.cur]
+    assert torch.equal(ta, tb)
+    assert torch.equal(a.TD, b.TD) and torch.equal(a.TDc, b.TDc)
+    shell = torch.ones((nz, n, n), dtype=torch.bool, device=ta.device)
+    shell[1:-1, 1:-1, 1:-1] = False
+    t0 = _random_state(p, n, nz, dtype, True, True, 9).T[0]
+    assert torch.equal(ta[shell], t0[shell])
+
+
 def test_offset_storage_is_exact_change_of_variables(orc):
     # FP64: storing T - T_art gives the same temperatures as kelvin storage
     n = 24
