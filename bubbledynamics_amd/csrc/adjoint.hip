@@ -148,13 +148,14 @@ __global__ void __launch_bounds__(kAX * kAY) adj_step_kernel(AdjArgs<T> a) {
             const size_t zo = (size_t)z * plane;
             const T w = q[3];
             const T* row = &s[b][ty + 3][tx + 3];
+            constexpr T i180 = T(1.0 / 180.0);
             const T Lx = (T(-490.0) * w + T(270.0) * (row[1] + row[-1]) - T(27.0) * (row[2] + row[-2]) +
-                          T(2.0) * (row[3] + row[-3])) / T(180.0);
+                          T(2.0) * (row[3] + row[-3])) * i180;
             const T Ly = (T(-490.0) * w + T(270.0) * (s[b][ty + 4][tx + 3] + s[b][ty + 2][tx + 3]) -
                           T(27.0) * (s[b][ty + 5][tx + 3] + s[b][ty + 1][tx + 3]) +
-                          T(2.0) * (s[b][ty + 6][tx + 3] + s[b][ty][tx + 3])) / T(180.0);
+                          T(2.0) * (s[b][ty + 6][tx + 3] + s[b][ty][tx + 3])) * i180;
             const T Lz = (T(-490.0) * w + T(270.0) * (q[4] + q[2]) - T(27.0) * (q[5] + q[1]) +
-                          T(2.0) * (q[6] + q[0])) / T(180.0);
+                          T(2.0) * (q[6] + q[0])) * i180;
             const T ST = a.kxy * (Lx + Ly) + a.kz * Lz;
             const T pn = cur.p0, pm1 = cur.p1, pm2 = cur.p2, pm3 = cur.p3;
             const T A = T(3.0) * pn - T(4.0) * pm1 + pm2;
