@@ -45,6 +45,7 @@ struct AdjArgs {
     int ocx, ocy, ocz;
     double or2, wT, wO;
     int ntx, nty, nzc, zchunk, nblocks;
+    int zb;                           // border columns of interior rows store their zeros
 };
 
 template <typename T>
@@ -75,7 +76,13 @@ struct AdjPlane {
 };
 
 // The adjoint field is stored as w = G lam (zero outside the interior), so
-// the sponge factor is formed once per cell when lam^n is written.
+// the sponge factor is formed once per cell when lam^n is written.  Tiles
+// start at x = 0, so the row segments a wavefront loads and stores are
+// line-aligned; with zb the lanes of the border columns x < 3 and x > N-4 of
+// an interior row store the zeros those cells hold (the buffers start
+// zeroed and no other write reaches them), so the seven stores cover whole
+// lines instead of leaving partly written ones at both ends of every row
+// (measured: 1 us of 32 at 128^3).
 template <typename T, int kAX, int kAY, int PF>
 __global__ void __launch_bounds__(kAX * kAY) adj_step_kernel(AdjArgs<T> a) {
     static_assert(PF == 1 || PF == 2, "planes of loads in flight: 1 or 2");
@@ -87,9 +94,10 @@ __global__ void __launch_bounds__(kAX * kAY) adj_step_kernel(AdjArgs<T> a) {
     const int tx = threadIdx.x % kAX, ty = threadIdx.x / kAX;
     const int N = a.N, Nz = a.Nz;
     const size_t plane = (size_t)N * N;
-    const int ix = 3 + bx * kAX + tx, iy = 3 + by * kAY + ty;
+    const int ix = bx * kAX + tx, iy = 3 + by * kAY + ty;
     const bool inr = ix < N && iy < N;
-    const bool act = ix <= N - 4 && iy <= N - 4;
+    const bool act = ix >= 3 && ix <= N - 4 && iy <= N - 4;
+    const bool zst = a.zb && inr && !act && iy <= N - 4;    // border column of an interior row
     const int z0 = 3 + zc * a.zchunk;
     const int z1 = min(z0 + a.zchunk, Nz - 3);
     constexpr uint32_t ES = sizeof(T);
@@ -99,12 +107,14 @@ __global__ void __launch_bounds__(kAX * kAY) adj_step_kernel(AdjArgs<T> a) {
     const bool xL = tx < 3, xR = tx >= kAX - 3, yB = ty < 3, yT = ty >= kAY - 3;
     const int hxo = xL ? -3 : 3;                       // x offset of this thread's halo cell
     // Halo cells beyond the grid are read only by threads outside the
-    // interior; their loads stay inside the field or return zero.
+    // interior; their loads stay inside the field or return zero.  An x halo
+    // left of x = 0 (first tile) is read by no interior lane and not loaded.
+    const bool hxin = ix + hxo >= 0;
     auto load_plane = [&](int z, AdjPlane<T>& v) {
         const size_t zo = (size_t)z * plane;
         const rsrc_t rw = make_rsrc(a.lam1 + zo, (size_t)(Nz - z) * plane * ES);
         v.wq = inr ? bload<T>(rw, o, 3 * pbytes) : T(0);
-        v.hx = (xL || xR) && iy < N ? bload<T>(rw, (uint32_t)((int)o + hxo * (int)ES)) : T(0);
+        v.hx = (xL || xR) && hxin && iy < N ? bload<T>(rw, (uint32_t)((int)o + hxo * (int)ES)) : T(0);
         v.hyb = yB && ix < N ? bload<T>(rw, (uint32_t)((int)o - 3 * N * (int)ES)) : T(0);
         v.hyt = yT && ix < N ? bload<T>(rw, (uint32_t)((int)o + 3 * N * (int)ES)) : T(0);
         if (act) {
@@ -176,6 +186,15 @@ __global__ void __launch_bounds__(kAX * kAY) adj_step_kernel(AdjArgs<T> a) {
             bstore<T>(T(3.0) * d, make_rsrc(a.acc4 + zo, pbytes), o);
             bstore<T>(rho1, make_rsrc(a.r1o + zo, pbytes), o);
             bstore<T>(rho2, make_rsrc(a.r2o + zo, pbytes), o);
+        } else if (zst) {
+            const size_t zo = (size_t)z * plane;
+            bstore<T>(T(0), make_rsrc(a.lam0 + zo, pbytes), o);
+            bstore<T>(T(0), make_rsrc(a.acc1 + zo, pbytes), o);
+            bstore<T>(T(0), make_rsrc(a.acc2 + zo, pbytes), o);
+            bstore<T>(T(0), make_rsrc(a.acc3 + zo, pbytes), o);
+            bstore<T>(T(0), make_rsrc(a.acc4 + zo, pbytes), o);
+            bstore<T>(T(0), make_rsrc(a.r1o + zo, pbytes), o);
+            bstore<T>(T(0), make_rsrc(a.r2o + zo, pbytes), o);
         }
 #pragma unroll
         for (int k = 0; k < 6; ++k) q[k] = q[k + 1];
@@ -376,12 +395,15 @@ static int adjoint_t(const bd_westervelt_params* prm, int Nxy, int Nz, int Nt, i
     // 64 or 128 columns wide, whichever leaves fewer idle lanes (BD_ADJ_TILE overrides)
     int tile = kAdjDefaultTile;
     {
-        const int nin = N - 6, w64 = (nin + 63) / 64 * 64 - nin, w128 = (nin + 127) / 128 * 128 - nin;
+        const int w64 = (N + 63) / 64 * 64 - N, w128 = (N + 127) / 128 * 128 - N;
         if (w128 <= w64) tile = kAdjWideTile;
     }
     if (const char* v = std::getenv("BD_ADJ_TILE")) { const int x = std::atoi(v); if (x >= 0 && x < 5) tile = x; }
     const int kAX = kAdjTiles[tile].ax, kAY = kAdjTiles[tile].ay;
-    A.ntx = (N - 6 + kAX - 1) / kAX;
+    // border zeros on (BD_ADJ_ZB=0 leaves the border columns unwritten, for measurement)
+    A.zb = 1;
+    if (const char* v = std::getenv("BD_ADJ_ZB")) A.zb = std::atoi(v) ? 1 : 0;
+    A.ntx = (N + kAX - 1) / kAX;
     A.nty = (N - 6 + kAY - 1) / kAY;
     {
         // target workgroup count (BD_ADJ_BLOCKS overrides it for the tile sweep)
@@ -389,7 +411,11 @@ static int adjoint_t(const bd_westervelt_params* prm, int Nxy, int Nz, int Nt, i
         if (const char* v = std::getenv("BD_ADJ_BLOCKS")) { const int x = std::atoi(v); if (x > 0) target = x; }
         const int nzin = Nz - 6;
         int nzc = std::max(1, target / (A.ntx * A.nty));
-        int zchunk = std::max(std::min(8, nzin), (nzin + nzc - 1) / nzc);
+        // chunks of at least 4 planes (128^3, 128 x 4 tile: 4-plane chunks
+        // 29.6 us per step, 8-plane 30.4, 16-plane 31.1; 6- and 12-plane
+        // chunks leave a partly filled last round of workgroups, 34.6 / 36.6)
+        int zchunk = std::max(std::min(4, nzin), (nzin + nzc - 1) / nzc);
+        if (const char* v = std::getenv("BD_ADJ_ZCHUNK")) { const int x = std::atoi(v); if (x > 0) zchunk = std::min(x, nzin); }
         A.nzc = (nzin + zchunk - 1) / zchunk;
         A.zchunk = zchunk;
         A.nblocks = A.ntx * A.nty * A.nzc;

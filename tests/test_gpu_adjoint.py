@@ -93,3 +93,23 @@ def test_every_adjoint_tile_same_gradient(monkeypatch, tile):
     assert J1 == J0
     assert np.abs(gA1 - gA0).max() <= 1e-12 * np.abs(gA0).max()
     assert np.abs(gP1 - gP0).max() <= 1e-12 * np.abs(gP0).max()
+
+
+@pytest.mark.parametrize("env", [{"BD_ADJ_ZB": "0"}, {"BD_ADJ_ZCHUNK": "7"}, {"BD_ADJ_ZCHUNK": "40"},
+                                 {"BD_ADJ_TILE": "1", "BD_ADJ_ZB": "0"}])
+def test_border_stores_and_z_chunks_do_not_change_gradient(monkeypatch, env):
+    # the zero stores of the border columns write the zeros those cells hold,
+    # and the z-chunk length only moves the per-workgroup boundaries of the
+    # backward march: the gradient is bit-identical (n = 67 leaves a partly
+    # filled last tile in x; 7-plane chunks a short last chunk)
+    n, steps = 67, 60
+    p, amp, ph, obj = setup(n, steps)
+    for k in ("BD_ADJ_ZB", "BD_ADJ_ZCHUNK", "BD_ADJ_TILE"):
+        monkeypatch.delenv(k, raising=False)
+    J0, gA0, gP0, _ = adjoint_gradient(p, n, steps, amp, ph, obj, 20, torch.float32)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    J1, gA1, gP1, _ = adjoint_gradient(p, n, steps, amp, ph, obj, 20, torch.float32)
+    assert J1 == J0
+    np.testing.assert_array_equal(gA1, gA0)
+    np.testing.assert_array_equal(gP1, gP0)
