@@ -14,7 +14,13 @@
 //   d^n = dlt w + a_1 rho_1^n + a_2 rho_2^n,   e^n = bet w
 //   lam^n  = acc[n] + 2w + 5d + e (3A/2 + B + 2p^n) + (kL)^T w + dJ/dp^n
 //   acc[n-1] += -w - 18d + e(-2A - 5p^n);  acc[n-2] += 24d + e(A/2 + 4p^n)
-//   acc[n-3] += -14d - e p^n;              acc[n-4]  = 3d
+//   acc[n-3]  = 3d^{n+1} + (-14d - e p^n)
+// (acc[n-3] starts with the 3 d^{n+1} that step n+1 contributes to it:
+// d^{n+1} = dlt w^{n+2} + a_1 rho_1^{n+1} + a_2 rho_2^{n+1} is formed again
+// here from w^{n+2} = G lam^{n+2}, the third level of the lam ring, and the
+// rho^{n+1} this step reads anyway -- the same operations on the same values
+// as step n+1, so the same bits -- instead of being stored by step n+1 and
+// read back: 19 words per cell instead of 20.)
 // At a source cell lam^n = (kL)^T w = kz (270 w(3) - 27 w(4) + 2 w(5)) / 180, and
 // dJ/dA_e = sum_n sum_{c in e} lam^n_c sin(w t_n + phi_e), dJ/dphi_e = ... A_e cos(...).
 // The forward levels a backward segment needs are recomputed from checkpoints
@@ -33,9 +39,9 @@ namespace bd {
 
 template <typename T>
 struct AdjArgs {
-    const T* lam1; T* lam0;
+    const T* lam1; T* lam0; const T* lam2;     // lam^{n+1} (stencil), lam^n (out), lam^{n+2}
     const T* p0; const T* p1; const T* p2; const T* p3;
-    const T* acc0; T* acc1; T* acc2; T* acc3; T* acc4;
+    const T* acc0; T* acc1; T* acc2; T* acc3;  // acc[n] (in), acc[n-1], acc[n-2] (in/out), acc[n-3] (out)
     const T* r1a; const T* r1b; T* r1o;
     const T* r2a; const T* r2b; T* r2o;
     T* lsrc;                          // lam^n on plane iz=2 (N*N), nullable
@@ -71,7 +77,8 @@ struct AdjPlane {
     T wq;                       // w at (ix, iy, z+3): the z queue
     T hx, hyb, hyt;             // x halo and bottom / top y halo of plane z (edge threads only)
     T p0, p1, p2, p3;           // p^n .. p^{n-3}
-    T c0, c1, c2, c3;           // acc[n] .. acc[n-3]
+    T c0, c1, c2;               // acc[n] .. acc[n-2]
+    T w2;                       // w^{n+2} = G lam^{n+2}
     T r1a, r1b, r2a, r2b;       // rho_v^{n+1}, rho_v^{n+2}
 };
 
@@ -80,7 +87,7 @@ struct AdjPlane {
 // start at x = 0, so the row segments a wavefront loads and stores are
 // line-aligned; with zb the lanes of the border columns x < 3 and x > N-4 of
 // an interior row store the zeros those cells hold (the buffers start
-// zeroed and no other write reaches them), so the seven stores cover whole
+// zeroed and no other write reaches them), so the six stores cover whole
 // lines instead of leaving partly written ones at both ends of every row
 // (measured: 1 us of 32 at 128^3).
 template <typename T, int kAX, int kAY, int PF>
@@ -125,7 +132,7 @@ __global__ void __launch_bounds__(kAX * kAY) adj_step_kernel(AdjArgs<T> a) {
             v.c0 = bload<T>(make_rsrc(a.acc0 + zo, pbytes), o);
             v.c1 = bload<T>(make_rsrc(a.acc1 + zo, pbytes), o);
             v.c2 = bload<T>(make_rsrc(a.acc2 + zo, pbytes), o);
-            v.c3 = bload<T>(make_rsrc(a.acc3 + zo, pbytes), o);
+            v.w2 = bload<T>(make_rsrc(a.lam2 + zo, pbytes), o);
             v.r1a = bload<T>(make_rsrc(a.r1a + zo, pbytes), o);
             v.r1b = bload<T>(make_rsrc(a.r1b + zo, pbytes), o);
             v.r2a = bload<T>(make_rsrc(a.r2a + zo, pbytes), o);
@@ -182,8 +189,8 @@ __global__ void __launch_bounds__(kAX * kAY) adj_step_kernel(AdjArgs<T> a) {
             bstore<T>(g * lam, make_rsrc(a.lam0 + zo, pbytes), o);
             bstore<T>(cur.c1 + (-w - T(18.0) * d + e * (T(-2.0) * A - T(5.0) * pn)), make_rsrc(a.acc1 + zo, pbytes), o);
             bstore<T>(cur.c2 + (T(24.0) * d + e * (T(0.5) * A + T(4.0) * pn)), make_rsrc(a.acc2 + zo, pbytes), o);
-            bstore<T>(cur.c3 + (T(-14.0) * d - e * pn), make_rsrc(a.acc3 + zo, pbytes), o);
-            bstore<T>(T(3.0) * d, make_rsrc(a.acc4 + zo, pbytes), o);
+            const T d_next = a.dlt * cur.w2 + a.a1 * cur.r1a + a.a2 * cur.r2a;   // d^{n+1}
+            bstore<T>(T(3.0) * d_next + (T(-14.0) * d - e * pn), make_rsrc(a.acc3 + zo, pbytes), o);
             bstore<T>(rho1, make_rsrc(a.r1o + zo, pbytes), o);
             bstore<T>(rho2, make_rsrc(a.r2o + zo, pbytes), o);
         } else if (zst) {
@@ -192,7 +199,6 @@ __global__ void __launch_bounds__(kAX * kAY) adj_step_kernel(AdjArgs<T> a) {
             bstore<T>(T(0), make_rsrc(a.acc1 + zo, pbytes), o);
             bstore<T>(T(0), make_rsrc(a.acc2 + zo, pbytes), o);
             bstore<T>(T(0), make_rsrc(a.acc3 + zo, pbytes), o);
-            bstore<T>(T(0), make_rsrc(a.acc4 + zo, pbytes), o);
             bstore<T>(T(0), make_rsrc(a.r1o + zo, pbytes), o);
             bstore<T>(T(0), make_rsrc(a.r2o + zo, pbytes), o);
         }
@@ -362,13 +368,17 @@ static int adjoint_t(const bd_westervelt_params* prm, int Nxy, int Nz, int Nt, i
 
     // ---- backward pass ----------------------------------------------------
     DevBuf lam, acc, rho, lsrc;
-    if (!lam.alloc(FB * 2) || !acc.alloc(FB * 5) || !rho.alloc(FB * 6) || !lsrc.alloc(plane * sizeof(T) * (size_t)(Nt + 1)))
+    if (!lam.alloc(FB * 3) || !acc.alloc(FB * 4) || !rho.alloc(FB * 6) || !lsrc.alloc(plane * sizeof(T) * (size_t)(Nt + 1)))
         return BD_ERR_ALLOC;
     T* L = (T*)lam.p;
     T* Ac = (T*)acc.p;
     T* Rh = (T*)rho.p;
-    auto Lv = [&](int m) { return L + (size_t)(m & 1) * NF; };
-    auto Av = [&](int m) { return Ac + (size_t)(((m % 5) + 5) % 5) * NF; };
+    // lam ring of 3 levels (lam^{Nt+1} is the zero slot the sweep starts
+    // with), accumulator ring of 4 (acc[m] is written at step m+3, updated at
+    // m+2 and m+1 and read at m; the levels above Nt-4 are never written and
+    // stay zero)
+    auto Lv = [&](int m) { return L + (size_t)(((m % 3) + 3) % 3) * NF; };
+    auto Av = [&](int m) { return Ac + (size_t)(((m % 4) + 4) % 4) * NF; };
     auto Rh1 = [&](int m) { return Rh + (size_t)(((m % 3) + 3) % 3) * NF; };
     auto Rh2 = [&](int m) { return Rh + (size_t)(3 + ((m % 3) + 3) % 3) * NF; };
     const WvCoef<T> cf = [&]() {
@@ -500,9 +510,9 @@ static int adjoint_t(const bd_westervelt_params* prm, int Nxy, int Nz, int Nt, i
         if (j != nseg - 1) BD_HIP_CHECK(hipStreamWaitEvent(st, recomputed[sl], 0));
         BD_HIP_CHECK(hipEventRecord(sweep_t[2 * j], st));
         for (int n = b - 1; n >= std::max(a0, 1); --n) {
-            A.lam1 = Lv(n + 1); A.lam0 = Lv(n);
+            A.lam1 = Lv(n + 1); A.lam0 = Lv(n); A.lam2 = Lv(n + 2);
             A.p0 = Sl(n, a0); A.p1 = Sl(n - 1, a0); A.p2 = Sl(n - 2, a0); A.p3 = Sl(n - 3, a0);
-            A.acc0 = Av(n); A.acc1 = Av(n - 1); A.acc2 = Av(n - 2); A.acc3 = Av(n - 3); A.acc4 = Av(n - 4);
+            A.acc0 = Av(n); A.acc1 = Av(n - 1); A.acc2 = Av(n - 2); A.acc3 = Av(n - 3);
             A.r1a = Rh1(n + 1); A.r1b = Rh1(n + 2); A.r1o = Rh1(n);
             A.r2a = Rh2(n + 1); A.r2b = Rh2(n + 2); A.r2o = Rh2(n);
             A.lsrc = (T*)lsrc.p + (size_t)n * plane;
