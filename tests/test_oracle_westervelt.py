@@ -295,3 +295,58 @@ def test_harmonics_match_numpy_fft(orc):
     expect = sum(2 * (7.2 * (k + 1) * p.fr / 1e6) * 2 * (amps[k] / 2) ** 2 / (p.rho_tissue * p.c_inf_tissue)
                  for k in range(n_harm))
     np.testing.assert_allclose(Q[izt:], expect, rtol=1e-9)
+
+
+def _nyquist_companion(p):
+    """One step of the linear scheme (beta = 0) for the grid's highest mode
+    (-1)^(x+y+z), whose 6th-order symbol is -1088/180 per axis, as a 9 x 9
+    matrix on (p^n..p^{n-4}, R1^{n-1}, R1^{n-2}, R2^{n-1}, R2^{n-2}): the
+    leapfrog, D3 (W:207) and the relaxation recurrence (W:226-230) written from
+    the parameters, independently of the oracle's code."""
+    dt, c = p.dt, p.c_inf_tissue
+    kL = ((c * dt / p.dxy) ** 2 * 2 + (c * dt / p.dz) ** 2) * (-1088.0 / 180.0)
+    dlt = p.delta_tissue / (2.0 * dt * c * c)
+    D3 = np.array([5.0, -18.0, 24.0, -14.0, 3.0, 0, 0, 0, 0])
+    rows = []
+    for cv, tv, k in ((p.c1, p.tau1, 5), (p.c2, p.tau2, 7)):
+        hist = np.zeros(9)
+        hist[k], hist[k + 1] = 4.0, -1.0
+        rows.append((2.0 * dt * 2.0 * cv * tv / ((2.0 * dt + 3.0 * tv) * c ** 3)) * D3 / (2.0 * dt ** 3)
+                    + tv / (2.0 * dt + 3.0 * tv) * hist)
+    M = np.zeros((9, 9))
+    M[0] = np.array([2.0 + kL, -1.0, 0, 0, 0, 0, 0, 0, 0]) + dlt * D3 + (c * dt) ** 2 * (rows[0] + rows[1])
+    M[1, 0] = M[2, 1] = M[3, 2] = M[4, 3] = 1.0
+    M[5], M[6, 5], M[7], M[8, 7] = rows[0], 1.0, rows[1], 1.0
+    return M
+
+
+def test_relaxation_scheme_amplifies_the_grid_nyquist_mode(orc):
+    # The relaxation discretisation of W:226-230 at the 192^3 tone-burst
+    # config (CFL 0.3, dt = 30 tau_1 = 3 tau_2) amplifies the grid's highest
+    # mode by 1.00103 per step, so content the burst leaves there grows ~6e5-fold
+    # in 13,000 steps (both precisions end non-finite after ~14,000 steps;
+    # the configured run is 3,000); without relaxation the scheme is neutral.
+    # The oracle's one-step map on that mode equals the companion matrix.
+    n, nz = 16, 16
+    p = presets.tone_burst_192(n)
+    p.beta = p.beta_tissue = 0.0
+    p.sponge_cells = 0
+    M = _nyquist_companion(p)
+    ev, V = np.linalg.eig(M)
+    i = int(np.argmax(np.abs(ev)))
+    assert abs(abs(ev[i]) - 1.001025) < 2e-6
+    q = p.copy()
+    q.c1 = q.c2 = 0.0
+    assert np.abs(np.linalg.eigvals(_nyquist_companion(q))).max() < 1.0 + 1e-9
+    v = np.real(V[:, i])
+    v /= np.abs(v).max()
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(n), np.arange(n), indexing="ij")
+    s = np.where((x + y + z) % 2 == 0, 1.0, -1.0)
+    lv = [v[k] * s for k in range(5)]
+    R1m1, R1m2, R2m1, R2m2 = (v[k] * s for k in (5, 6, 7, 8))
+    Pn, R1, R2 = np.zeros((nz, n, n)), np.zeros((nz, n, n)), np.zeros((nz, n, n))
+    orc.wv_space6(p, n, nz, Pn, *lv, R1, R1m1, R1m2, R2, R2m1, R2m2)
+    w = M @ v
+    sl = (slice(3, nz - 3), slice(0, n - 3), slice(0, n - 3))
+    for got, k in ((Pn, 0), (R1, 5), (R2, 7)):
+        np.testing.assert_allclose(got[sl], (w[k] * s)[sl], rtol=1e-11, atol=1e-11 * abs(w[k]))
