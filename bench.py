@@ -485,6 +485,50 @@ def threshold_table(first: int, deadline: float, out_dir: str, reference_tableau
             "threshold_mpa_median": float(np.median(found) / 1e6) if found.size else None}
 
 
+def threshold_table_bounded(first: int, deadline: float, out_dir: str, reference_tableau: bool,
+                            budget: int) -> dict:
+    """The same table from cell `first` on with the resumable sweep: launches
+    of at most `budget` DP5(4) attempts per lane, every launch's finished cells
+    saved (out_dir/sweep_partial_<first>.npy, NaN where a cell is not done),
+    and no launch started after `deadline` seconds, so the run ends within one
+    launch of the deadline whatever its cells cost."""
+    import torch
+    from bubbledynamics_amd import gilmore as gm
+    p = gm.reference_params()
+    p.tableau = 0 if reference_tableau else 1
+    fr = gm.freq_table()
+    n_r0 = 100
+    total = len(fr) * n_r0
+    os.makedirs(out_dir, exist_ok=True)
+    sw = gm.ThresholdSweep(p, fr, n_r0, cell0=first)
+    t0 = time.perf_counter()
+    pr = {}
+    while True:
+        sw.advance(budget)
+        torch.cuda.synchronize()
+        pr = sw.progress()
+        el = time.perf_counter() - t0
+        part = sw.results[first:].cpu().numpy()
+        np.save(os.path.join(out_dir, f"sweep_partial_{first}.npy"), part)
+        done = int(np.sum(~np.isnan(part)))
+        log(f"sweep: launch {sw.advances}, {done} of {total - first} cells done, {pr['suspended']} suspended, "
+            f"{el:.0f} s")
+        if pr["all_done"] or el > deadline > 0:
+            break
+    secs = time.perf_counter() - t0
+    sw.close()
+    found = part[(part > 0) & (part < 1e7)]
+    info = {"workload": "threshold_sweep_bounded", "tableau": "reference" if reference_tableau else
+            "Dormand-Prince 1980", "cells": total, "first": first, "done": done, "all_done": bool(pr["all_done"]),
+            "launches": sw.advances, "budget_attempts_per_lane": budget, "seconds": round(secs, 1),
+            "no_collapse_1e7": int(np.sum(part == 1e7)), "capped_minus2": int(np.sum(part == -2.0)),
+            "unset_minus1": int(np.sum(part == -1.0)),
+            "threshold_mpa_median": float(np.median(found) / 1e6) if found.size else None}
+    with open(os.path.join(out_dir, f"sweep_partial_{first}.json"), "w") as f:
+        json.dump(info, f)
+    return info
+
+
 def assemble_table(out_dir: str, path: str) -> dict:
     """Joins the sweep pieces of out_dir into the reference's table file."""
     import glob
@@ -543,6 +587,9 @@ def main() -> int:
                     help="seconds after which the sweep stops claiming cells (0: run to the end); "
                          "cells already claimed still run to their end")
     ap.add_argument("--sweep-out", default="sweep_parts", help="directory of the sweep pieces")
+    ap.add_argument("--sweep-budget", type=int, default=0,
+                    help="resumable sweep: DP5(4) attempts per lane per launch (0: one launch, claimed cells "
+                         "run to their end); the finished cells of every launch are saved")
     ap.add_argument("--sweep-reference-tableau", action="store_true",
                     help="the reference's a32 = -9/40 instead of the published Dormand-Prince tableau")
     ap.add_argument("--sweep-assemble", default="",
@@ -566,6 +613,11 @@ def main() -> int:
         print("no GPU visible", file=sys.stderr)
         return 3
     _ffi.load()
+    if args.threshold_sweep and args.sweep_budget > 0:
+        res = threshold_table_bounded(args.sweep_first, args.sweep_deadline, args.sweep_out,
+                                      args.sweep_reference_tableau, args.sweep_budget)
+        print(json.dumps({"n_gpus": 1, **res}), flush=True)
+        return 0
     if args.threshold_sweep:
         res = threshold_table(args.sweep_first, args.sweep_deadline, args.sweep_out, args.sweep_reference_tableau)
         print(json.dumps({"n_gpus": 1, **res}), flush=True)

@@ -261,3 +261,23 @@ def test_conditioning_fixture_spot_check(orc):
         return max(abs(run(R0[i] * (1 + u), amp[i]) / b - 1), abs(run(R0[i], amp[i] * (1 + u)) / b - 1))
     assert spread(worst) > 1e-9
     assert spread(calm) <= 1e-9
+
+
+def test_reference_tableau_table_samples_match_oracle(orc):
+    # cells of the threshold table with the reference's own tableau and
+    # tolerances as the GPU's drop-in sweep computed them (fixture written by
+    # tests/golden/make_reference_tableau_samples.py from a bounded
+    # `bench.py --threshold-sweep --sweep-reference-tableau` run); two of the
+    # eight are recomputed here (about 7 s each on one core): the oracle's
+    # search gives the same amplitude, bit for bit
+    import json
+    from pathlib import Path
+    from bubbledynamics_amd import gilmore as gm
+    fx = json.loads((Path(__file__).parent / "golden" / "reference_tableau_samples.json").read_text())
+    p = gm.reference_params()
+    assert p.tableau == 0
+    fr = gm.freq_table()
+    n_r0 = fx["n_r0"]
+    for idx, want in list(zip(fx["cells"], fx["gpu_threshold_pa"]))[::4]:
+        got = orc.gilmore_threshold(p, fr, n_r0, idx, idx + 1, nthreads=1)[idx]
+        assert got == want, (idx, fr[idx // n_r0], got, want)
