@@ -139,6 +139,61 @@ def gilmore_cpu_baseline(budget_s: float) -> dict:
             "dp45_subset_rmax": r3["Rmax"].tolist(), "dp45_subset_collapsed": (r3["t_collapse"] > 0).tolist()}
 
 
+def bioheat_cpu_baseline(n: int, budget_s: float) -> dict:
+    """FP64 oracle Pennes substeps with the Kahan-compensated dose (T:52-105)
+    on the host, on a bounded sample of the coupled 256^3 workload (a uniform
+    heat source; the work per cell does not depend on it): all threads of the
+    share, then one thread; rates in cell-substeps/s."""
+    import oracle
+    from bubbledynamics_amd import bioheat as bh
+    orc = oracle.load()
+    p = bh.pennes_params(dx=2e-4, dt=0.05, substeps=2)
+    p.T_off = 0.0
+    Ta = np.full((n, n, n), p.T_art)
+    Tb = Ta.copy()
+    Q = np.full((n, n, n), 1e6)
+    TD = np.zeros((n, n, n))
+    TDc = np.zeros((n, n, n))
+
+    def rate(threads, budget):
+        orc.temperature_square(p, Ta, Tb, Q, TD, n, n, 0, 0, 0, TDc, threads)   # warm-up
+        t0 = time.perf_counter()
+        done = 0
+        while time.perf_counter() - t0 < budget or done < 2:
+            orc.temperature_square(p, Ta, Tb, Q, TD, n, n, (done + 1) % 2, 0, 0, TDc, threads)
+            done += 1
+        el = time.perf_counter() - t0
+        return (n - 2) ** 3 * done / el, done, el
+
+    cores = cpu_threads()
+    v_all, d_all, e_all = rate(cores, budget_s)
+    v_one, d_one, e_one = rate(1, budget_s / 2)
+    return {"value": v_all, "unit": "cell-substeps/s", "cores": cores, "kind": "port", "serial_1core": v_one,
+            "sample": f"FP64 oracle with the compensated dose, {n}^3: {d_all} substeps on {cores} threads in "
+                      f"{e_all:.1f} s and {d_one} on 1 thread in {e_one:.1f} s, each after 1 warm-up substep"}
+
+
+def adjoint_cpu_baseline(p, n: int, amp, ph, budget_s: float) -> dict:
+    """FP64 oracle forward steps of the 128^3 phased-array workload on the
+    host (all threads of the share): the CPU path of the gradient is the
+    forward solve repeated per central finite difference (two solves per
+    parameter), timed on a bounded prefix of steps and projected."""
+    import oracle
+    orc = oracle.load()
+    state = np.zeros((12, n, n, n))
+    cores = cpu_threads()
+    orc.wv_run(p, n, n, 0, 1, state, None, None, None, (-1, -1, -1), amp, ph, cores)   # warm-up
+    t0 = time.perf_counter()
+    done = 0
+    while time.perf_counter() - t0 < budget_s or done < 2:
+        orc.wv_run(p, n, n, 1 + done, 1, state, None, None, None, (-1, -1, -1), amp, ph, cores)
+        done += 1
+    el = time.perf_counter() - t0
+    return {"value": n ** 3 * done / el / 1e6, "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle, {n}^3 phased array: {done} forward steps on {cores} threads in {el:.1f} s "
+                      "after 1 warm-up step"}
+
+
 def read_traffic(profile: Path, kernel: str) -> float | None:
     """HBM bytes per launch of the step kernel `kernel` from the committed
     counter summary (line 'hbm_bytes_per_launch <kernel>: N'), or None."""
@@ -241,10 +296,14 @@ def gilmore_figures(cpu_budget_s: float | None) -> dict:
     return out
 
 
-def adjoint_figures() -> dict:
+def adjoint_figures(cpu_budget_s: float | None = None) -> dict:
     """128^3, 64-element array, 1,500 steps, FP32; checkpoints every 50 steps and
     every level kept; cost (forward + backward phase wall time) as a multiple of
-    the forward solve.  The segment recompute overlaps the adjoint sweep."""
+    the forward solve.  The segment recompute overlaps the adjoint sweep.  With
+    cpu_budget_s, the FP64 oracle's forward rate on the host and the projected
+    host time of the central-difference gradient the configuration checks (8
+    elements, amplitude and phase: 32 forward solves) beside the GPU gradient of
+    all 64 elements."""
     import math
     import torch
     from bubbledynamics_amd import presets
@@ -262,10 +321,20 @@ def adjoint_figures() -> dict:
         out[f"checkpoint_{K}"] = {"forward_ms": tm[0], "recompute_outside_sweep_ms": tm[1], "sweep_ms": tm[2],
                                   "cost_x_forward": (tm[0] + tm[1] + tm[2]) / tm[0], "J": J,
                                   "grad_amp_norm": float(np.linalg.norm(gA))}
+    if cpu_budget_s is not None:
+        cpu = adjoint_cpu_baseline(p, n, amp, ph, cpu_budget_s)
+        fwd_s = n ** 3 * steps / (cpu["value"] * 1e6)
+        grad_gpu_s = sum(out["checkpoint_50"][k] for k in ("forward_ms", "recompute_outside_sweep_ms",
+                                                           "sweep_ms")) / 1e3
+        cpu["forward_solve_s_projected"] = fwd_s
+        cpu["fd_gradient_8_elements_s_projected"] = 32 * fwd_s
+        cpu["gpu_gradient_64_elements_s"] = grad_gpu_s
+        cpu["gpu_forward_x"] = fwd_s / (out["checkpoint_50"]["forward_ms"] / 1e3)
+        out["cpu_baseline"] = cpu
     return out
 
 
-def coupled_figures() -> dict:
+def coupled_figures(cpu_budget_s: float | None = None) -> dict:
     """256^3 CW Westervelt -> Q = 2 alpha I (5 Np/m/MHz) -> Pennes, 49-point spiral, 1 s dwell."""
     import torch
     from bubbledynamics_amd import workloads as wl
@@ -280,7 +349,7 @@ def coupled_figures() -> dict:
     # measured traffic per pair is in the committed counter summary
     traffic = read_traffic(ROOT / "profiles" / "bioheat_pair_256_trace.txt", "bh_march2")
     pair_cells = 2 * cells
-    return {"westervelt_steps": tq["steps"], "westervelt_s": tq["settle_s"] + tq["window_s"],
+    out = {"westervelt_steps": tq["steps"], "westervelt_s": tq["settle_s"] + tq["window_s"],
             "q_max_w_m3": float(Q.max().item()), "bioheat_substeps": nsub, "bioheat_s": secs,
             "bioheat_mcell_steps_per_s": rate / 1e6,
             "bioheat_alg_bytes_per_cell_substep": 14,
@@ -289,6 +358,14 @@ def coupled_figures() -> dict:
             "bioheat_single_substep_equiv_gb_s_28B": rate * 28 / 1e9,
             "peak_rise_k": float((T.max() - run.p.T_art).item()),
             "max_cem43_min": float(run.dose().max().item() / 60.0)}
+    if cpu_budget_s is not None:
+        del run, T
+        torch.cuda.empty_cache()
+        cpu = bioheat_cpu_baseline(n, cpu_budget_s)
+        cpu["gpu_x"] = rate / cpu["value"]
+        cpu["gpu_x_serial"] = rate / cpu["serial_1core"]
+        out["bioheat_cpu_baseline"] = cpu
+    return out
 
 
 def reference_scale(shifts: int, compare: int = -1, out_dir: str | None = None, thermal_f64: bool = False,
@@ -752,8 +829,8 @@ def main() -> int:
 
     if not args.no_extras:
         gil_cpu = None if args.no_cpu else args.cpu_seconds
-        for name, fn in (("gilmore", lambda: gilmore_figures(gil_cpu)), ("adjoint", adjoint_figures),
-                         ("coupled_bioheat", coupled_figures)):
+        for name, fn in (("gilmore", lambda: gilmore_figures(gil_cpu)), ("adjoint", lambda: adjoint_figures(gil_cpu)),
+                         ("coupled_bioheat", lambda: coupled_figures(gil_cpu))):
             try:
                 log(f"{name} ...")
                 extra[name] = fn()
