@@ -14,7 +14,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -379,6 +381,233 @@ __global__ void __launch_bounds__(BX * TY) bh_march2(BhArgs<T> a) {
     }
 }
 
+// Three substeps in one launch: bh_march2 with one more level.  The BX x
+// TY*RY region runs substep 1 on every region cell, substep 2 one plane
+// behind and substep 3 two planes behind; the output is the inner
+// (BX-4) x (TY*RY-4) cells, and neighbouring regions overlap by four
+// columns / rows (the region starts one cell left of and below its output
+// block, so the first region reaches the column and row -1, which no kept
+// value reads).  z: the workgroup owns output planes [z0, z1) and runs
+// iterations zs = max(z0-2, 0) .. z1+1: substep 1 at zs (when zs <= Nz-1),
+// substep 2 at zs-1 (planes z0-1 .. z1 that exist), substep 3 at zs-2
+// (output planes).  Substep-1 and substep-2 values live in registers (own
+// column, 3 planes each) and in LDS planes (x/y neighbours); the dose of the
+// three substeps is added in order before TD is stored once.  A cell that
+// a substep does not update (grid boundary planes, rows and columns) keeps
+// its value at every level, read from `dst`: three single substeps
+// alternate between the fields, whose boundary shells bd_bioheat_run makes
+// equal, and the result goes to `dst`, where the third of them would leave
+// it.  The per-cell arithmetic is that of three single substeps, so the
+// result is bit-identical to them.
+template <typename T, int BX, int TY, int RY, bool KAHAN>
+__global__ void __launch_bounds__(BX * TY) bh_march3(BhArgs<T> a) {
+    constexpr int ROWS = TY * RY;
+    static_assert(TY >= 2 && RY >= 1 && BX % 64 == 0, "two thread rows of whole wavefronts");
+    static_assert(ROWS >= 6, "region too short for three substeps");
+    constexpr uint32_t OOB = 0xFFFFFFFFu;
+    constexpr bool kahan = KAHAN;
+    __shared__ T sa[2][ROWS + 2][BX + 2];    // T^n plane zs with a one-cell halo
+    __shared__ T sb[2][ROWS + 2][BX + 2];    // substep-1 plane (its ring is never read into a kept value)
+    __shared__ T sc[2][ROWS + 2][BX + 2];    // substep-2 plane (likewise)
+    const int lb = xcd_remap((int)blockIdx.x, a.nblocks);
+    const int zc = lb / (a.ntx * a.nty), t2i = lb % (a.ntx * a.nty);   // tile-major
+    const int bx = t2i % a.ntx, by = t2i / a.ntx;
+    const int tx = (int)threadIdx.x % BX, ty = (int)threadIdx.x / BX;   // ty < TY
+    const int N = a.Nxy;
+    const size_t plane = (size_t)N * N;
+    const uint32_t ES = sizeof(T);
+    const uint32_t pbytes = (uint32_t)(plane * ES);
+    const int gx = bx * (BX - 4) + tx - 1;
+    const int gy0 = by * (ROWS - 4) - 1;
+    const int z0 = 1 + zc * a.zchunk;
+    const int z1 = min(z0 + a.zchunk, a.Nz - 1);
+    const BhDoseC<T> dc{a.dose_off, a.T_off, a.dt_T};
+    __shared__ T scoef[2][6];
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const BhCoef<T> c = bh_coef(a, m);
+            scoef[m][0] = c.keep; scoef[m][1] = c.kxy; scoef[m][2] = c.kz;
+            scoef[m][3] = c.dq; scoef[m][4] = c.perf; scoef[m][5] = c.theta;
+        }
+    }
+    auto coef = [&](int m) {
+        return BhCoef<T>{scoef[m][0], scoef[m][1], scoef[m][2], scoef[m][3], scoef[m][4], scoef[m][5]};
+    };
+
+    bool itr[RY];
+    uint32_t o_in[RY], o_edge[RY], o_out[RY], cq[RY], ohx[RY];
+    const int hxd = tx == 0 ? -1 : (tx == BX - 1 ? 1 : 0);
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        const int ly = ty * RY + r;
+        const int gy = gy0 + ly;
+        const bool ing = gx >= 0 && gx < N && gy >= 0 && gy < N;
+        itr[r] = gx >= 1 && gx <= N - 2 && gy >= 1 && gy <= N - 2;
+        const bool outc = itr[r] && tx >= 2 && tx <= BX - 3 && ly >= 2 && ly <= ROWS - 3;
+        const uint32_t o = ing ? ((uint32_t)gy * (uint32_t)N + (uint32_t)gx) * ES : OOB;
+        o_in[r] = o;
+        o_edge[r] = ing && !itr[r] ? o : OOB;
+        o_out[r] = outc ? o : OOB;
+        const int ixa = gx - a.shift_x, iya = gy - a.shift_y;
+        const bool q_in = itr[r] && ixa >= 0 && ixa < N && iya >= 0 && iya < N;   // T:76
+        cq[r] = q_in ? ((uint32_t)iya * (uint32_t)N + (uint32_t)ixa) * ES : OOB;
+        const int hx = gx + hxd;
+        ohx[r] = (hxd != 0 && gy >= 0 && gy < N && hx >= 0 && hx < N)
+                     ? ((uint32_t)gy * (uint32_t)N + (uint32_t)hx) * ES : OOB;
+    }
+    const int hyd = ty == 0 ? -1 : (ty == TY - 1 ? 1 : 0);
+    const int hyrow = hyd < 0 ? gy0 - 1 : gy0 + ROWS;
+    const uint32_t ohy = (hyd != 0 && gx >= 0 && gx < N && hyrow >= 0 && hyrow < N)
+                             ? ((uint32_t)hyrow * (uint32_t)N + (uint32_t)gx) * ES : OOB;
+
+    struct P3 { T zp[RY], q[RY], td[RY], tdc[RY], bnd[RY], hx[RY], hy; };
+    // loads of substep-1 plane zs (0 <= zs <= Nz-1); zp is plane zs+1 (0 past the last plane)
+    auto load = [&](int zs, P3& v) {
+        const size_t zo = (size_t)zs * plane;
+        const rsrc_t rs = make_rsrc(a.src + zo, pbytes);
+        const rsrc_t rn = make_rsrc(a.src + zo + plane, zs + 1 < a.Nz ? pbytes : 0u);
+        const rsrc_t rq = make_rsrc(a.Q + zo, pbytes);
+        const rsrc_t rb = make_rsrc(a.dst + zo, pbytes);
+        const rsrc_t rd = make_rsrc(a.TD + zo, pbytes);
+        const bool bplane = zs == 0 || zs == a.Nz - 1;
+        const bool outp = zs >= z0 && zs < z1;
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const uint32_t od = outp ? o_out[r] : OOB;
+            const uint32_t ob = bplane ? o_in[r] : o_edge[r];
+            v.zp[r] = bload<T>(rn, o_in[r]);
+            v.q[r] = bload<T>(rq, cq[r]);
+            v.bnd[r] = ob != OOB ? bload<T>(rb, ob) : T(0);
+            v.td[r] = bload<T>(rd, od);
+            v.hx[r] = ohx[r] != OOB ? bload<T>(rs, ohx[r]) : T(0);
+        }
+        if constexpr (KAHAN) {
+            const rsrc_t rc = make_rsrc(a.TDc + zo, pbytes);
+#pragma unroll
+            for (int r = 0; r < RY; ++r) v.tdc[r] = bload<T>(rc, outp ? o_out[r] : OOB);
+        } else {
+#pragma unroll
+            for (int r = 0; r < RY; ++r) v.tdc[r] = T(0);
+        }
+        v.hy = ohy != OOB ? bload<T>(rs, ohy) : T(0);
+    };
+
+    const int zb = max(z0 - 2, 0), ze = z1 + 1;     // iterations
+    const int last1 = min(ze, a.Nz - 1);             // last substep-1 plane
+    const int lo2 = max(z0 - 1, 0), hi2 = min(z1, a.Nz - 1);
+    // own column: T^n at zs-1, zs; substep 1 at zs-2, zs-1; substep 2 at zs-3,
+    // zs-2; Q at zs-1, zs-2; boundary values of plane zs-1; dose after
+    // substep 1 (plane zs-1) and after substep 2 (plane zs-2)
+    T tm[RY], tc[RY], s1m[RY], s1c[RY], s2m[RY], s2c[RY], qp[RY], qpp[RY], bndp[RY];
+    T h1[RY], h1c[RY], h2[RY], h2c[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        const rsrc_t rs = make_rsrc(a.src + (size_t)zb * plane, pbytes);
+        const rsrc_t rm = make_rsrc(a.src + (size_t)(zb > 0 ? zb - 1 : 0) * plane, zb >= 1 ? pbytes : 0u);
+        tm[r] = bload<T>(rm, o_in[r]);
+        tc[r] = bload<T>(rs, o_in[r]);
+        s1m[r] = s1c[r] = s2m[r] = s2c[r] = qp[r] = qpp[r] = bndp[r] = T(0);
+        h1[r] = h1c[r] = h2[r] = h2c[r] = T(0);
+    }
+
+    P3 cur, nxt;
+    load(zb, cur);
+    for (int zs = zb; zs <= ze; ++zs) {
+        if (zs + 1 <= last1) load(zs + 1, nxt);
+        const int b = (zs - zb) & 1;
+        const bool do1 = zs <= a.Nz - 1;
+        if (do1) {
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                const int lp = ty * RY + r + 1;
+                sa[b][lp][tx + 1 + hxd] = cur.hx[r];   // halo column on edge lanes (own cell otherwise,
+                sa[b][lp][tx + 1] = tc[r];             // overwritten by this lane's next write)
+            }
+            if (hyd < 0) sa[b][0][tx + 1] = cur.hy;
+            if (hyd > 0) sa[b][ROWS + 1][tx + 1] = cur.hy;
+        }
+        __syncthreads();
+        // substep 1 at plane zs on the region
+        T s1[RY];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) s1[r] = T(0);
+        if (do1) {
+            const bool upd = !(zs == 0 || zs == a.Nz - 1);
+            const BhCoef<T> c1 = coef(zs >= a.iz_tissue ? 1 : 0);
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                const int lp = ty * RY + r + 1;
+                const T u = bh_update_c(c1, tc[r], sa[b][lp][tx], sa[b][lp][tx + 2], sa[b][lp - 1][tx + 1],
+                                        sa[b][lp + 1][tx + 1], cur.zp[r], tm[r], cur.q[r]);
+                s1[r] = (upd && itr[r]) ? u : cur.bnd[r];
+                sb[b][lp][tx + 1] = s1[r];
+            }
+        }
+        // substep 2 at plane zs-1 on the region (its neighbours: sb[b^1], written
+        // by the previous iteration before this iteration's barrier)
+        const int p2 = zs - 1;
+        T s2[RY];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) s2[r] = T(0);
+        if (p2 >= lo2 && p2 <= hi2) {
+            const bool upd = !(p2 == 0 || p2 == a.Nz - 1);
+            const BhCoef<T> c2 = coef(p2 >= a.iz_tissue ? 1 : 0);
+            const int pb = b ^ 1;
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                const int lp = ty * RY + r + 1;
+                T v = bndp[r];
+                if (upd) {
+                    const T u = bh_update_c(c2, s1c[r], sb[pb][lp][tx], sb[pb][lp][tx + 2], sb[pb][lp - 1][tx + 1],
+                                            sb[pb][lp + 1][tx + 1], s1[r], s1m[r], qp[r]);
+                    if (itr[r]) v = u;
+                }
+                s2[r] = v;
+                sc[b][lp][tx + 1] = v;
+            }
+        }
+        // substep 3 at plane zs-2 on the output cells (neighbours: sc[b^1])
+        const int p3 = zs - 2;
+        if (p3 >= z0 && p3 < z1) {
+            const BhCoef<T> c3 = coef(p3 >= a.iz_tissue ? 1 : 0);
+            const size_t zo = (size_t)p3 * plane;
+            const int pb = b ^ 1;
+            const rsrc_t rdst = make_rsrc(a.dst + zo, pbytes);
+            const rsrc_t rd = make_rsrc(a.TD + zo, pbytes);
+            const rsrc_t rc = make_rsrc(KAHAN ? a.TDc + zo : a.TD + zo, KAHAN ? pbytes : 0u);
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                const int lp = ty * RY + r + 1;
+                const T T3 = bh_update_c(c3, s2c[r], sc[pb][lp][tx], sc[pb][lp][tx + 2], sc[pb][lp - 1][tx + 1],
+                                         sc[pb][lp + 1][tx + 1], s2[r], s2m[r], qpp[r]);
+                T td = h2[r], tdc = h2c[r];
+                bh_accumulate(td, tdc, bh_dose_c(dc, T3), kahan);
+                bstore<T>(T3, rdst, o_out[r]);
+                if constexpr (KAHAN) bstore<T>(tdc, rc, o_out[r]);
+                bstore<T>(td, rd, o_out[r]);
+            }
+        }
+        // doses held until the next substep reaches the plane (lanes and
+        // planes without an output cell hold values never stored)
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            T d2 = h1[r], d2c = h1c[r];
+            bh_accumulate(d2, d2c, bh_dose_c(dc, s2[r]), kahan);
+            h2[r] = d2; h2c[r] = d2c;
+            T d1 = cur.td[r], d1c = cur.tdc[r];
+            bh_accumulate(d1, d1c, bh_dose_c(dc, s1[r]), kahan);
+            h1[r] = d1; h1c[r] = d1c;
+            s2m[r] = s2c[r]; s2c[r] = s2[r];
+            s1m[r] = s1c[r]; s1c[r] = s1[r];
+            qpp[r] = qp[r]; qp[r] = cur.q[r];
+            bndp[r] = cur.bnd[r];
+            tm[r] = tc[r]; tc[r] = cur.zp[r];
+        }
+        if (zs + 1 <= last1) cur = nxt;
+    }
+}
+
 static int first_tissue_plane_bh(const bd_bioheat_params* p, int Nz) {
     for (int iz = 0; iz < Nz; ++iz)
         if (!(iz * p->dz < p->water_tissue)) return iz;   // T:79
@@ -504,9 +733,76 @@ static int bh2_variant(int Nxy) {
     return v;
 }
 
+// Region shapes of the triple kernel (FP32): BX columns (BX - 4 output
+// columns), TY thread rows of two rows each (2 TY - 4 output rows).
+struct Bh3Variant { int bx, ty; };
+constexpr Bh3Variant kBh3Variants[] = {{64, 16}, {128, 8}};
+constexpr int kBh3NumVariants = (int)(sizeof(kBh3Variants) / sizeof(kBh3Variants[0]));
+
+template <int BX, int TY, bool KAHAN>
+static int run_bh3(BhArgs<float> a, int variant, hipStream_t st) {
+    constexpr int ROWS = TY * 2;
+    const int nin = a.Nxy - 2;
+    a.ntx = (nin + (BX - 4) - 1) / (BX - 4);
+    a.nty = (nin + (ROWS - 4) - 1) / (ROWS - 4);
+    const int nzin = a.Nz - 2;
+    // z-chunk length: each workgroup runs zchunk + 3 iterations; the length
+    // of least rounds * (zchunk + 3), at least 8 planes (BD_BH_ZCHUNK overrides)
+    constexpr int kMaxDev = 64;
+    static int resident[kMaxDev][2][kBh3NumVariants] = {};
+    static std::mutex mtx;
+    int dev = 0, res = 0;
+    BD_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDev) return BD_ERR_INVALID;
+    {
+        std::lock_guard<std::mutex> lock(mtx);
+        res = resident[dev][KAHAN][variant];
+    }
+    if (res == 0) {
+        int per_cu = 0;
+        hipDeviceProp_t prop;
+        BD_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+        BD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bh_march3<float, BX, TY, 2, KAHAN>,
+                                                                  BX * TY, 0));
+        res = std::max(1, per_cu) * prop.multiProcessorCount;
+        std::lock_guard<std::mutex> lock(mtx);
+        resident[dev][KAHAN][variant] = res;
+    }
+    const long long tiles = (long long)a.ntx * a.nty;
+    int zchunk = nzin;
+    long long best = -1;
+    for (int zc = std::min(8, nzin); zc <= nzin; ++zc) {
+        const int nzc = (nzin + zc - 1) / zc;
+        if (zc > std::min(8, nzin) && (nzin + zc - 2) / (zc - 1) == nzc) continue;   // same split, longer chunk
+        const long long rounds = (tiles * nzc + res - 1) / res;
+        const long long cost = rounds * (zc + 3);
+        if (best < 0 || cost < best) { best = cost; zchunk = zc; }
+    }
+    if (const char* v = std::getenv("BD_BH_ZCHUNK")) { const int x = std::atoi(v); if (x > 0) zchunk = std::min(x, nzin); }
+    a.nzc = (nzin + zchunk - 1) / zchunk;
+    a.zchunk = zchunk;
+    a.nblocks = (int)(tiles * a.nzc);
+    hipLaunchKernelGGL((bh_march3<float, BX, TY, 2, KAHAN>), dim3(a.nblocks), dim3(BX * TY), 0, st, a);
+    BD_LAUNCH_CHECK();
+    return BD_OK;
+}
+
+static BhArgs<float> bh_fused_args(const bd_bioheat_params* p, const float* src, float* dst, const float* Q,
+                                   float* TD, float* TDc, int Nxy, int Nz, int sx, int sy);
+
+// The triple kernel with region shape `variant` (index into kBh3Variants).
+static int launch_bh3(const bd_bioheat_params* p, const float* src, float* dst, const float* Q, float* TD,
+                      float* TDc, int Nxy, int Nz, int sx, int sy, int variant, hipStream_t st) {
+    const BhArgs<float> a = bh_fused_args(p, src, dst, Q, TD, TDc, Nxy, Nz, sx, sy);
+    if (variant == 0) return TDc ? run_bh3<64, 16, true>(a, 0, st) : run_bh3<64, 16, false>(a, 0, st);
+    return TDc ? run_bh3<128, 8, true>(a, 1, st) : run_bh3<128, 8, false>(a, 1, st);
+}
+
+// Arguments of the fused kernels: bh_march2 and bh_march3 in the library's
+// expression order.
 template <typename T>
-static int launch_bh2(const bd_bioheat_params* p, const T* src, T* dst, const T* Q, T* TD, T* TDc, int Nxy,
-                      int Nz, int sx, int sy, hipStream_t st) {
+static BhArgs<T> bh_fused_args_t(const bd_bioheat_params* p, const T* src, T* dst, const T* Q, T* TD, T* TDc,
+                                 int Nxy, int Nz, int sx, int sy) {
     BhArgs<T> a;
     a.src = src; a.dst = dst; a.Q = Q; a.TD = TD; a.TDc = TDc;
     a.Nxy = Nxy; a.Nz = Nz;
@@ -523,6 +819,18 @@ static int launch_bh2(const bd_bioheat_params* p, const T* src, T* dst, const T*
     a.dt_T = p->dt_T;
     a.dose_off = (T)((p->T_off - 273.15) - 43.0);
     a.reference_order = 0;
+    return a;
+}
+
+static BhArgs<float> bh_fused_args(const bd_bioheat_params* p, const float* src, float* dst, const float* Q,
+                                   float* TD, float* TDc, int Nxy, int Nz, int sx, int sy) {
+    return bh_fused_args_t<float>(p, src, dst, Q, TD, TDc, Nxy, Nz, sx, sy);
+}
+
+template <typename T>
+static int launch_bh2(const bd_bioheat_params* p, const T* src, T* dst, const T* Q, T* TD, T* TDc, int Nxy,
+                      int Nz, int sx, int sy, hipStream_t st) {
+    const BhArgs<T> a = bh_fused_args_t<T>(p, src, dst, Q, TD, TDc, Nxy, Nz, sx, sy);
     const int v = bh2_variant<T>(Nxy);
     if constexpr (sizeof(T) == 4) {
         switch (v) {
@@ -620,22 +928,92 @@ extern "C" int bd_bioheat_run(const bd_bioheat_params* prm, int dtype, void* T_a
                                (double*)f[1 - w], Nxy, Nz);
         BD_LAUNCH_CHECK();
     }
-    // pairs of substeps in one launch (BD_BH_FUSE=0 runs them one at a time).
+    // three substeps in one launch (FP32), then pairs, then single substeps
+    // (BD_BH_FUSE=2: pairs and single substeps; 0: single substeps only).
     // 256^3, 1,960 substeps (G cell-substeps/s, single / paired):
     // FP32 + Kahan 153 / 222, FP64 87 / 149.
-    bool fuse = true;
-    if (const char* v = std::getenv("BD_BH_FUSE")) fuse = std::atoi(v) != 0;
+    int fuse = dtype == BD_F32 ? 3 : 2;
+    if (const char* v = std::getenv("BD_BH_FUSE")) {
+        const int x = std::atoi(v);
+        if (x == 0 || x == 2) fuse = std::min(fuse, x);
+    }
     int k = 0;
-    for (; fuse && k + 2 <= n_steps; k += 2) {
+    auto triple = [&](int v) {
         const int w = *which;
-        if (dtype == BD_F32)
-            rc = launch_bh2<float>(prm, (const float*)f[w], (float*)f[1 - w], (const float*)Q, (float*)TD,
-                                   (float*)TD_comp, Nxy, Nz, shift_x, shift_y, st);
-        else
-            rc = launch_bh2<double>(prm, (const double*)f[w], (double*)f[1 - w], (const double*)Q, (double*)TD,
-                                    (double*)TD_comp, Nxy, Nz, shift_x, shift_y, st);
+        const int r = launch_bh3(prm, (const float*)f[w], (float*)f[1 - w], (const float*)Q, (float*)TD,
+                                 (float*)TD_comp, Nxy, Nz, shift_x, shift_y, v, st);
+        if (!r) *which = 1 - w;
+        return r;
+    };
+    auto pair = [&]() {
+        const int w = *which;
+        const int r = dtype == BD_F32
+            ? launch_bh2<float>(prm, (const float*)f[w], (float*)f[1 - w], (const float*)Q, (float*)TD,
+                                (float*)TD_comp, Nxy, Nz, shift_x, shift_y, st)
+            : launch_bh2<double>(prm, (const double*)f[w], (double*)f[1 - w], (const double*)Q, (double*)TD,
+                                 (double*)TD_comp, Nxy, Nz, shift_x, shift_y, st);
+        if (!r) *which = 1 - w;
+        return r;
+    };
+    // FP32: triples (either region shape) or pairs, whichever the grid runs
+    // faster (their results are the same bits): measured once per device,
+    // grid and dose mode on the first call with at least 11 substeps -- a
+    // warm-up triple, then one launch of each timed with events -- and kept
+    // for the process.  Measured (us per substep, pairs / 64 x 32 / 128 x 16
+    // triples): 256^3 46 / 62 / 80, 426^2 x 300 201 / 162 / 193, 851^2 x 200
+    // 482 / 431 / 391, 1024^2 x 100 390 / 320 / 357.  BD_BH_VARIANT3 forces
+    // the triple shape.
+    int mode = -1;                      // 0, 1: triples of that shape; 2: pairs
+    if (fuse >= 3) {
+        if (const char* e = std::getenv("BD_BH_VARIANT3")) {
+            const int x = std::atoi(e);
+            if (x >= 0 && x < kBh3NumVariants) mode = x;
+        }
+    }
+    if (fuse >= 3 && mode < 0) {
+        static std::mutex mtx;
+        static std::map<std::tuple<int, int, int, int>, int> chosen;
+        int dev = 0;
+        BD_HIP_CHECK(hipGetDevice(&dev));
+        const auto key = std::make_tuple(dev, Nxy, Nz, TD_comp ? 1 : 0);
+        {
+            std::lock_guard<std::mutex> lock(mtx);
+            const auto it = chosen.find(key);
+            if (it != chosen.end()) mode = it->second;
+        }
+        if (mode < 0 && n_steps - k >= 11) {
+            hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+            for (auto& e : ev) BD_HIP_CHECK(hipEventCreate(&e));
+            int r = triple(0);                                   // warm-up
+            if (!r) r = hipEventRecord(ev[0], st) == hipSuccess ? BD_OK : BD_ERR_HIP;
+            if (!r) r = triple(0);
+            if (!r) r = hipEventRecord(ev[1], st) == hipSuccess ? BD_OK : BD_ERR_HIP;
+            if (!r) r = triple(1);
+            if (!r) r = hipEventRecord(ev[2], st) == hipSuccess ? BD_OK : BD_ERR_HIP;
+            if (!r) r = pair();
+            if (!r) r = hipEventRecord(ev[3], st) == hipSuccess ? BD_OK : BD_ERR_HIP;
+            if (!r) r = hipEventSynchronize(ev[3]) == hipSuccess ? BD_OK : BD_ERR_HIP;
+            float t[3] = {0.f, 0.f, 0.f};
+            for (int i = 0; i < 3 && !r; ++i)
+                r = hipEventElapsedTime(&t[i], ev[i], ev[i + 1]) == hipSuccess ? BD_OK : BD_ERR_HIP;
+            for (auto& e : ev) (void)hipEventDestroy(e);
+            if (r) return r;
+            k += 11;
+            const float per[3] = {t[0] / 3.f, t[1] / 3.f, t[2] / 2.f};
+            mode = 0;
+            for (int i = 1; i < 3; ++i) if (per[i] < per[mode]) mode = i;
+            std::lock_guard<std::mutex> lock(mtx);
+            chosen[key] = mode;
+        }
+        if (mode < 0) mode = 2;
+    }
+    for (; fuse >= 3 && mode < 2 && k + 3 <= n_steps; k += 3) {
+        rc = triple(mode);
         if (rc) return rc;
-        *which = 1 - w;
+    }
+    for (; fuse >= 2 && k + 2 <= n_steps; k += 2) {
+        rc = pair();
+        if (rc) return rc;
     }
     for (; k < n_steps; ++k) {
         const int w = *which;

@@ -140,13 +140,42 @@ def test_every_pair_region_bitwise_equal(monkeypatch, dtype, kahan, variant):
     b = _random_state(p, n, nz, dtype, kahan, kahan, 5)
     for sx, sy, m in ((2, -3, 6), (-9, 4, 3)):
         monkeypatch.setenv("BD_BH_VARIANT", str(variant))
-        monkeypatch.delenv("BD_BH_FUSE", raising=False)
+        monkeypatch.setenv("BD_BH_FUSE", "2")
         a.step(sx, sy, m)
         monkeypatch.delenv("BD_BH_VARIANT", raising=False)
         monkeypatch.setenv("BD_BH_FUSE", "0")
         b.step(sx, sy, m)
     monkeypatch.delenv("BD_BH_FUSE", raising=False)
     torch.cuda.synchronize()
+    assert torch.equal(a.T[a.cur], b.T[b.cur])
+    assert torch.equal(a.TD, b.TD)
+    if kahan:
+        assert torch.equal(a.TDc, b.TDc)
+
+
+@pytest.mark.parametrize("variant", range(2))
+@pytest.mark.parametrize("n,nz,kahan", [(131, 29, True), (70, 41, False), (37, 13, True)])
+def test_every_triple_region_bitwise_equal(monkeypatch, variant, n, nz, kahan):
+    # each region shape of the FP32 triple kernel (64 x 32 and 128 x 16
+    # regions, 60 x 28 and 124 x 12 output) against single substeps: grids
+    # that end inside a region in x and y and inside a z-chunk, two media,
+    # perfusion, shifted source, substep counts that leave a pair or a single
+    # substep after the triples
+    p = two_layer(n, nz)
+    p.T_art = 310.15
+    p.perf = 2e-3
+    a = _random_state(p, n, nz, torch.float32, kahan, kahan, 7)
+    b = _random_state(p, n, nz, torch.float32, kahan, kahan, 7)
+    for sx, sy, m in ((2, -3, 9), (-9, 4, 5), (0, 0, 4), (5, 5, 3)):
+        monkeypatch.setenv("BD_BH_VARIANT3", str(variant))
+        monkeypatch.delenv("BD_BH_FUSE", raising=False)
+        a.step(sx, sy, m)
+        monkeypatch.delenv("BD_BH_VARIANT3", raising=False)
+        monkeypatch.setenv("BD_BH_FUSE", "0")
+        b.step(sx, sy, m)
+    monkeypatch.delenv("BD_BH_FUSE", raising=False)
+    torch.cuda.synchronize()
+    assert not torch.equal(a.T[a.cur], a.T[1 - a.cur])
     assert torch.equal(a.T[a.cur], b.T[b.cur])
     assert torch.equal(a.TD, b.TD)
     if kahan:
