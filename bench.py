@@ -368,6 +368,48 @@ def coupled_figures(cpu_budget_s: float | None = None) -> dict:
     return out
 
 
+def bioheat_shapes_figures() -> dict:
+    """FP32 Pennes substeps with the compensated dose at several grid widths:
+    pairs against the two region shapes of the three-substep kernel (forced
+    through BD_BH_FUSE / BD_BH_VARIANT3, which the library reads on every
+    call), and the choice bd_bioheat_run measures by itself; us per substep,
+    best of three runs of 60 substeps after a 6-substep warm-up."""
+    import torch
+    from bubbledynamics_amd import bioheat as bh
+    p = bh.pennes_params(dx=2e-4, dt=0.05, substeps=2)
+    out = {}
+    for n, nz in ((256, 256), (426, 300), (851, 200), (1024, 100)):
+        Q = torch.zeros((nz, n, n), dtype=torch.float32, device="cuda")
+        Q[nz // 2, n // 2, n // 2] = 1e9
+        row = {}
+        for name, env in (("pairs", {"BD_BH_FUSE": "2"}), ("triples_64x32", {"BD_BH_VARIANT3": "0"}),
+                          ("triples_128x16", {"BD_BH_VARIANT3": "1"}), ("measured_choice", {})):
+            saved = {k: os.environ.pop(k, None) for k in ("BD_BH_FUSE", "BD_BH_VARIANT3")}
+            os.environ.update(env)
+            try:
+                run = bh.BioheatRun(p, n, nz, Q, p.T_art, dtype=torch.float32, kahan=True, offset=True)
+                run.step(0, 0, 6)
+                torch.cuda.synchronize()
+                best = 1e9
+                for _ in range(3):
+                    t0 = time.perf_counter()
+                    run.step(0, 0, 60)
+                    torch.cuda.synchronize()
+                    best = min(best, time.perf_counter() - t0)
+                row[name] = round(best / 60 * 1e6, 2)
+                del run
+            finally:
+                for k in ("BD_BH_FUSE", "BD_BH_VARIANT3"):
+                    os.environ.pop(k, None)
+                    if saved[k] is not None:
+                        os.environ[k] = saved[k]
+        out[f"{n}x{n}x{nz}_us_per_substep"] = row
+        del Q
+        torch.cuda.empty_cache()
+        log(f"bioheat shapes {n}x{n}x{nz}: {row}")
+    return out
+
+
 def reference_scale(shifts: int, compare: int = -1, out_dir: str | None = None, thermal_f64: bool = False,
                     npy_dir: str | None = None) -> dict:
     """The reference's heat-source and thermal programs at their own grids."""
@@ -640,7 +682,7 @@ def main() -> int:
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the Gilmore, adjoint and coupled bioheat figures")
-    ap.add_argument("--figure", choices=("gilmore", "adjoint", "coupled"), default=None,
+    ap.add_argument("--figure", choices=("gilmore", "adjoint", "coupled", "bioheat_shapes"), default=None,
                     help="run only that auxiliary workload and print its JSON (for profiling)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first copy a buffer of this size 3 times (HBM counter calibration)")
@@ -713,7 +755,7 @@ def main() -> int:
         del src, dst
     if args.figure:
         fn = {"gilmore": lambda: gilmore_figures(None), "adjoint": adjoint_figures,
-              "coupled": coupled_figures}[args.figure]
+              "coupled": coupled_figures, "bioheat_shapes": bioheat_shapes_figures}[args.figure]
         print(json.dumps({"figure": args.figure, "n_gpus": 1, **fn()}), flush=True)
         return 0
     n, K, W = args.n, args.steps, args.warmup
