@@ -955,15 +955,17 @@ extern "C" int bd_bioheat_run(const bd_bioheat_params* prm, int dtype, void* T_a
         if (!r) *which = 1 - w;
         return r;
     };
-    // FP32: triples (either region shape) or pairs, whichever the grid runs
+    // FP32: triples (any region shape) or pairs, whichever the grid runs
     // faster (their results are the same bits): measured once per device,
     // grid and dose mode on the first call with at least 11 substeps -- a
     // warm-up triple, then one launch of each timed with events -- and kept
     // for the process.  Measured (us per substep, pairs / 64 x 32 / 128 x 16
     // triples): 256^3 46 / 62 / 80, 426^2 x 300 201 / 162 / 193, 851^2 x 200
-    // 482 / 431 / 391, 1024^2 x 100 390 / 320 / 357.  BD_BH_VARIANT3 forces
-    // the triple shape.
-    int mode = -1;                      // 0, 1: triples of that shape; 2: pairs
+    // 482 / 431 / 391, 1024^2 x 100 390 / 320 / 357; a 64 x 16 region (512
+    // threads) ran 60 / 177 / 430 / 324 and won nowhere.  BD_BH_VARIANT3
+    // forces the triple shape.
+    constexpr int kPairs = kBh3NumVariants;
+    int mode = -1;                      // < kPairs: triples of that shape; kPairs: pairs
     if (fuse >= 3) {
         if (const char* e = std::getenv("BD_BH_VARIANT3")) {
             const int x = std::atoi(e);
@@ -981,33 +983,34 @@ extern "C" int bd_bioheat_run(const bd_bioheat_params* prm, int dtype, void* T_a
             const auto it = chosen.find(key);
             if (it != chosen.end()) mode = it->second;
         }
-        if (mode < 0 && n_steps - k >= 11) {
-            hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-            for (auto& e : ev) BD_HIP_CHECK(hipEventCreate(&e));
-            int r = triple(0);                                   // warm-up
-            if (!r) r = hipEventRecord(ev[0], st) == hipSuccess ? BD_OK : BD_ERR_HIP;
-            if (!r) r = triple(0);
-            if (!r) r = hipEventRecord(ev[1], st) == hipSuccess ? BD_OK : BD_ERR_HIP;
-            if (!r) r = triple(1);
-            if (!r) r = hipEventRecord(ev[2], st) == hipSuccess ? BD_OK : BD_ERR_HIP;
-            if (!r) r = pair();
-            if (!r) r = hipEventRecord(ev[3], st) == hipSuccess ? BD_OK : BD_ERR_HIP;
-            if (!r) r = hipEventSynchronize(ev[3]) == hipSuccess ? BD_OK : BD_ERR_HIP;
-            float t[3] = {0.f, 0.f, 0.f};
-            for (int i = 0; i < 3 && !r; ++i)
-                r = hipEventElapsedTime(&t[i], ev[i], ev[i + 1]) == hipSuccess ? BD_OK : BD_ERR_HIP;
-            for (auto& e : ev) (void)hipEventDestroy(e);
+        constexpr int kProbe = 3 + 3 * kBh3NumVariants + 2;    // warm-up triple, each shape, a pair
+        if (mode < 0 && n_steps - k >= kProbe) {
+            hipEvent_t ev[kPairs + 2] = {};
+            int r = BD_OK;
+            for (auto& e : ev) if (!r) r = hipEventCreate(&e) == hipSuccess ? BD_OK : BD_ERR_HIP;
+            if (!r) r = triple(0);                               // warm-up
+            for (int i = 0; i <= kPairs && !r; ++i) {
+                r = hipEventRecord(ev[i], st) == hipSuccess ? BD_OK : BD_ERR_HIP;
+                if (!r) r = i < kPairs ? triple(i) : pair();
+            }
+            if (!r) r = hipEventRecord(ev[kPairs + 1], st) == hipSuccess ? BD_OK : BD_ERR_HIP;
+            if (!r) r = hipEventSynchronize(ev[kPairs + 1]) == hipSuccess ? BD_OK : BD_ERR_HIP;
+            float per[kPairs + 1] = {};
+            for (int i = 0; i <= kPairs && !r; ++i) {
+                r = hipEventElapsedTime(&per[i], ev[i], ev[i + 1]) == hipSuccess ? BD_OK : BD_ERR_HIP;
+                per[i] /= i < kPairs ? 3.f : 2.f;
+            }
+            for (auto& e : ev) if (e) (void)hipEventDestroy(e);
             if (r) return r;
-            k += 11;
-            const float per[3] = {t[0] / 3.f, t[1] / 3.f, t[2] / 2.f};
+            k += kProbe;
             mode = 0;
-            for (int i = 1; i < 3; ++i) if (per[i] < per[mode]) mode = i;
+            for (int i = 1; i <= kPairs; ++i) if (per[i] < per[mode]) mode = i;
             std::lock_guard<std::mutex> lock(mtx);
             chosen[key] = mode;
         }
-        if (mode < 0) mode = 2;
+        if (mode < 0) mode = kPairs;
     }
-    for (; fuse >= 3 && mode < 2 && k + 3 <= n_steps; k += 3) {
+    for (; fuse >= 3 && mode < kPairs && k + 3 <= n_steps; k += 3) {
         rc = triple(mode);
         if (rc) return rc;
     }
