@@ -820,9 +820,8 @@ def main() -> int:
     achieved = alg_bytes / (step_ms / 1e3)
     log(f"step kernel {step_ms * 1e3:.2f} us per step in the timed region ({k_ms * 1e3:.2f} us relaunched alone) "
         f"-> {achieved / 1e9:.0f} GB/s algorithmic; shape {shape}")
-    kname = "wv_march<float, %d, %d, %d, %d, %s>" % (
-        shape["tile_x"], shape["tile_y"], shape["rows"], shape["prefetch"],
-        "true" if shape["nontemporal"] else "false")
+    kname = "wv_march<float, %d, %d, %d, %d, %d>" % (
+        shape["tile_x"], shape["tile_y"], shape["rows"], shape["prefetch"], shape["nontemporal"])
     del run32
     torch.cuda.empty_cache()
 

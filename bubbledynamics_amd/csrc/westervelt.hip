@@ -331,6 +331,17 @@ static bool streams_past_mall(int Nxy, int Nz, size_t elem_bytes) {
     return (size_t)12 * Nxy * Nxy * Nz * elem_bytes > kMallBytes;
 }
 
+// History-load cache policy without a measurement (0 cached, 1 every level
+// non-temporal, 2 only p^{n-3}, p^{n-4}, R^{n-2}): non-temporal once the
+// 12-field state outgrows the MALL; BD_WV_NT forces one.
+static int nt_rule(int Nxy, int Nz, size_t elem_bytes) {
+    if (const char* e = std::getenv("BD_WV_NT")) {
+        const int x = std::atoi(e);
+        if (x >= 0 && x <= 2) return x;
+    }
+    return streams_past_mall(Nxy, Nz, elem_bytes) ? 1 : 0;
+}
+
 static int check_params(const bd_westervelt_params* p, int dtype, int Nxy, int Nz) {
     if (!p || (dtype != BD_F32 && dtype != BD_F64)) return BD_ERR_INVALID;
     if (Nxy < 10 || Nz < 10) return BD_ERR_INVALID;
@@ -409,11 +420,14 @@ static WvSrc make_src(const bd_westervelt_params* p, const double* amp, const do
 template <typename T, int V>
 static void launch_variant(const WvMarchArgs<T>& a, int step_arg, hipStream_t st) {
     constexpr WvVariant v = kWvVariants[V];
-    if (a.nt)
-        hipLaunchKernelGGL((wv_march<T, v.tx, v.ty, v.ry, v.pf, true>), dim3(a.g.nblocks),
+    if (a.nt == 1)
+        hipLaunchKernelGGL((wv_march<T, v.tx, v.ty, v.ry, v.pf, 1>), dim3(a.g.nblocks),
+                           dim3(v.tx * v.ty), 0, st, a, step_arg);
+    else if (a.nt == 2)
+        hipLaunchKernelGGL((wv_march<T, v.tx, v.ty, v.ry, v.pf, 2>), dim3(a.g.nblocks),
                            dim3(v.tx * v.ty), 0, st, a, step_arg);
     else
-        hipLaunchKernelGGL((wv_march<T, v.tx, v.ty, v.ry, v.pf, false>), dim3(a.g.nblocks),
+        hipLaunchKernelGGL((wv_march<T, v.tx, v.ty, v.ry, v.pf, 0>), dim3(a.g.nblocks),
                            dim3(v.tx * v.ty), 0, st, a, step_arg);
 }
 
@@ -483,8 +497,7 @@ static void set_fused_z(WvMarchArgs<T>& a, const bd_westervelt_params* p, int on
     a.kz_w = p->c_inf * p->dt / p->dz;
     a.kz_t = p->c_inf_tissue * p->dt / p->dz;
     a.fuse_xy = on;
-    // non-temporal history loads once the 12-field state outgrows the MALL
-    a.nt = streams_past_mall(a.g.Nxy, a.g.Nz, sizeof(T)) ? 1 : 0;
+    a.nt = nt_rule(a.g.Nxy, a.g.Nz, sizeof(T));   // a plan's measured policy replaces it
     a.kxy_w = p->c_inf * p->dt / p->dxy;
     a.kxy_t = p->c_inf_tissue * p->dt / p->dxy;
     a.d_time = nullptr;
@@ -700,6 +713,7 @@ struct bd_westervelt_plan {
     int target_blocks = 0;              // workgroup target of make_geom
     int epart_blocks = 0;               // per-step capacity of d_epart (largest candidate tile)
     bool tuned = false;                 // the step tile was chosen by measurement (prepare)
+    int nt = 0;                         // history-load cache policy: nt_rule at creation, measured by prepare
 };
 
 extern "C" size_t bd_westervelt_state_elems(int Nxy, int Nz) {
@@ -719,6 +733,7 @@ extern "C" int bd_westervelt_plan_create(const bd_westervelt_params* prm, int dt
     pl->prm = *prm; pl->dtype = dtype; pl->Nxy = Nxy; pl->Nz = Nz; pl->max_steps = max_steps;
     int blocks = 0;
     launch_shape(prm, Nxy, Nz, dtype == BD_F64, &pl->variant, &blocks);
+    pl->nt = nt_rule(Nxy, Nz, dtype == BD_F64 ? 8 : 4);
     pl->target_blocks = blocks;
     pl->g = make_geom(prm, Nxy, Nz, pl->variant, blocks);
     pl->epart_blocks = pl->g.nblocks;
@@ -787,6 +802,7 @@ static int plan_step(bd_westervelt_plan* pl, int n, const int* d_step, int step_
     a.g = pl->g;
     a.c = make_coef<T>(&pl->prm);
     set_fused_z<T>(a, &pl->prm, 1, pl->amp, pl->phase);
+    a.nt = pl->nt;
     if (d_step) a.d_time = pl->d_time;            // graph node: t = d_time[*d_step + j + 1]
     else a.t_next = pl->h_time[(size_t)n + 1];
     return launch_march<T>(a, pl->variant, step_arg, st);
@@ -880,12 +896,14 @@ extern "C" int bd_westervelt_plan_run(bd_westervelt_plan* pl, int n_steps, int u
 }
 
 
-// Launches the step kernel of the next step again with tile `variant` on
-// geometry g (state not advanced; it rewrites the next levels, the step's
-// energy partials and focus sample, which the real step overwrites, and
-// with use_pmax the peak-pressure field).
+// Launches the step kernel of the next step again with tile `variant` and
+// history cache policy `nt` (-1: the plan's) on geometry g (state not
+// advanced; it rewrites the next levels, the step's energy partials and focus
+// sample, which the real step overwrites, and with use_pmax the peak-pressure
+// field).
 template <typename T>
-static int plan_rerun_step_t(bd_westervelt_plan* pl, hipStream_t st, int variant, const WvGeom& g, bool use_pmax) {
+static int plan_rerun_step_t(bd_westervelt_plan* pl, hipStream_t st, int variant, const WvGeom& g, bool use_pmax,
+                             int nt = -1) {
     const int n = pl->steps;
     const size_t N = (size_t)pl->Nxy * pl->Nxy * pl->Nz;
     T* S = (T*)pl->state;
@@ -906,6 +924,8 @@ static int plan_rerun_step_t(bd_westervelt_plan* pl, hipStream_t st, int variant
     a.g = g;
     a.c = make_coef<T>(&pl->prm);
     set_fused_z<T>(a, &pl->prm, 1, pl->amp, pl->phase);
+    if (nt >= 0) a.nt = nt;
+    else a.nt = pl->nt;
     a.t_next = pl->h_time[(size_t)n + 1];
     return launch_march<T>(a, variant, n, st);
 }
@@ -916,14 +936,20 @@ extern "C" int bd_westervelt_plan_rerun_step(bd_westervelt_plan* pl, void* strea
     return plan_rerun_step_t<double>(pl, as_stream(stream), pl->variant, pl->g, true);
 }
 
-// Chooses the step tile by measurement: each candidate relaunches the next
-// step (2 untimed + 12 timed launches, pmax untouched, nothing the real step
-// does not overwrite) on a private stream; the fastest mean is kept.  Runs
-// once per plan, before its first step; a BD_WV_VARIANT override skips it.
+// Chooses the step tile and the history cache policy by measurement: each
+// candidate relaunches the next step (2 untimed + 12 timed launches, pmax
+// untouched, nothing the real step does not overwrite) on a private stream;
+// the fastest mean is kept.  Policies: with a state past the last-level
+// cache every history level non-temporal or only p^{n-3}, p^{n-4} and
+// R^{n-2} (FP32 192^3, 340 MB: 66.9 / 66.0 us per step; 256^3, 805 MB:
+// 163 / 186); with a state it holds, cached or the same partial policy.
+// Runs once per plan, before its first step; a BD_WV_VARIANT or BD_WV_NT
+// override skips it.
 template <typename T>
 static int plan_tune_t(bd_westervelt_plan* pl) {
     pl->tuned = true;
-    if (std::getenv("BD_WV_VARIANT") || pl->steps != 0 || pl->steps >= pl->max_steps) return BD_OK;
+    if (std::getenv("BD_WV_VARIANT") || std::getenv("BD_WV_NT") || pl->steps != 0 || pl->steps >= pl->max_steps)
+        return BD_OK;
     const bool f32 = std::is_same<T, float>::value;
     const int* cands = f32 ? kTuneF32 : kTuneF64;
     const int nc = f32 ? (int)(sizeof(kTuneF32) / sizeof(int)) : (int)(sizeof(kTuneF64) / sizeof(int));
@@ -939,23 +965,27 @@ static int plan_tune_t(bd_westervelt_plan* pl) {
     BD_HIP_CHECK(hipEventCreate(&r.e0));
     BD_HIP_CHECK(hipEventCreate(&r.e1));
     constexpr int kWarm = 2, kTimed = 12;
-    int best = pl->variant;
+    const int pol[2] = {streams_past_mall(pl->Nxy, pl->Nz, sizeof(T)) ? 1 : 0, 2};
+    int best = pl->variant, best_nt = pol[0];
     float best_ms = 1e30f;
     for (int i = 0; i < nc; ++i) {
         const WvGeom g = make_geom(&pl->prm, pl->Nxy, pl->Nz, cands[i], pl->target_blocks);
         if (g.nblocks > pl->epart_blocks) continue;
-        for (int k = 0; k < kWarm + kTimed; ++k) {
-            if (k == kWarm) BD_HIP_CHECK(hipEventRecord(r.e0, r.st));
-            const int rc = plan_rerun_step_t<T>(pl, r.st, cands[i], g, false);
-            if (rc) return rc;
+        for (const int nt : pol) {
+            for (int k = 0; k < kWarm + kTimed; ++k) {
+                if (k == kWarm) BD_HIP_CHECK(hipEventRecord(r.e0, r.st));
+                const int rc = plan_rerun_step_t<T>(pl, r.st, cands[i], g, false, nt);
+                if (rc) return rc;
+            }
+            BD_HIP_CHECK(hipEventRecord(r.e1, r.st));
+            BD_HIP_CHECK(hipEventSynchronize(r.e1));
+            float ms = 0.f;
+            BD_HIP_CHECK(hipEventElapsedTime(&ms, r.e0, r.e1));
+            if (ms < best_ms) { best_ms = ms; best = cands[i]; best_nt = nt; }
         }
-        BD_HIP_CHECK(hipEventRecord(r.e1, r.st));
-        BD_HIP_CHECK(hipEventSynchronize(r.e1));
-        float ms = 0.f;
-        BD_HIP_CHECK(hipEventElapsedTime(&ms, r.e0, r.e1));
-        if (ms < best_ms) { best_ms = ms; best = cands[i]; }
     }
     pl->variant = best;
+    pl->nt = best_nt;
     pl->g = make_geom(&pl->prm, pl->Nxy, pl->Nz, best, pl->target_blocks);
     return BD_OK;
 }
@@ -983,12 +1013,11 @@ extern "C" int bd_westervelt_plan_shape(const bd_westervelt_plan* pl, int* tile_
                                         int* prefetch, int* nontemporal, int* workgroups) {
     if (!pl) return BD_ERR_INVALID;
     const WvVariant v = kWvVariants[pl->variant];
-    const size_t es = pl->dtype == BD_F64 ? 8 : 4;
     if (tile_x) *tile_x = v.tx;
     if (tile_y) *tile_y = v.ty;
     if (rows) *rows = v.ry;
     if (prefetch) *prefetch = v.pf;
-    if (nontemporal) *nontemporal = streams_past_mall(pl->Nxy, pl->Nz, es) ? 1 : 0;
+    if (nontemporal) *nontemporal = pl->nt;
     if (workgroups) *workgroups = pl->g.nblocks;
     return BD_OK;
 }

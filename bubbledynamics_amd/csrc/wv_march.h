@@ -7,9 +7,10 @@
 //     buffered: one barrier per plane);
 //   * z-neighbours come from a 7-deep register queue per column fed by one
 //     coalesced load of plane z+3;
-//   * p^{n-1..n-4}, R1/R2 history and pmax are streamed once (the history
-//     levels with the non-temporal cache policy when the state is larger
-//     than the 256 MB last-level cache, which otherwise keeps it);
+//   * p^{n-1..n-4}, R1/R2 history and pmax are streamed once (when the state
+//     is larger than the 256 MB last-level cache, every history level or only
+//     the two oldest pressure levels and the older relaxation level with the
+//     non-temporal cache policy, whichever the plan measures faster);
 //   * with PF, all loads of plane z+1 are issued before plane z is computed.
 // Fused outputs: pmax, per-block energy partials, the focus sample; with
 // fuse_z the source plane and the z faces; with fuse_xy the x/y Mur faces,
@@ -110,7 +111,7 @@ struct WvMarchArgs {
     int fuse_z, boundary;
     double kz_w, kz_t;         // Courant numbers of the z faces (W:100, W:111)
     int fuse_xy;               // x/y faces after the march (requires fuse_z)
-    int nt;                    // launch the instantiation with non-temporal history loads
+    int nt;                    // history loads: 0 cached, 1 all non-temporal, 2 p^{n-3}, p^{n-4}, R^{n-2} non-temporal
     double kxy_w, kxy_t;       // Courant numbers of the x/y faces (W:117-119)
     const double* d_time;      // time table (graph replay), t = d_time[step+1]
     double t_next;             // t^{n+1} when d_time is null
@@ -142,7 +143,7 @@ struct PlaneIn {
     T r1m1[RY], r1m2[RY], r2m1[RY], r2m2[RY], pmx[RY];
 };
 
-template <typename T, int TX, int TY, int RY, int PF, bool NTL>
+template <typename T, int TX, int TY, int RY, int PF, int NTM>
 __global__ void __launch_bounds__(TX * TY)
 wv_march(WvMarchArgs<T> a, int step_arg) {
     constexpr int NT = TX * TY;
@@ -212,8 +213,11 @@ wv_march(WvMarchArgs<T> a, int step_arg) {
             v.qn[r] = bload<T>(rp, o, 3 * pbytes);
             v.hl[r] = xL ? bload<T>(rp, o + (uint32_t)(dxl * (int)ES)) : T(0);
             v.hr[r] = xR ? bload<T>(rp, o + 3 * ES) : T(0);
-            if constexpr (NTL) {
+            if constexpr (NTM == 1) {
                 v.pm1[r] = bload_nt<T>(q1, o); v.pm2[r] = bload_nt<T>(q2, o);
+                v.pm3[r] = bload_nt<T>(q3, o); v.pm4[r] = bload_nt<T>(q4, o);
+            } else if constexpr (NTM == 2) {
+                v.pm1[r] = bload<T>(q1, o); v.pm2[r] = bload<T>(q2, o);
                 v.pm3[r] = bload_nt<T>(q3, o); v.pm4[r] = bload_nt<T>(q4, o);
             } else {
                 v.pm1[r] = bload<T>(q1, o); v.pm2[r] = bload<T>(q2, o);
@@ -226,9 +230,12 @@ wv_march(WvMarchArgs<T> a, int step_arg) {
 #pragma unroll
             for (int r = 0; r < RY; ++r) {
                 const uint32_t o = cidx[r] * ES;
-                if constexpr (NTL) {
+                if constexpr (NTM == 1) {
                     v.r1m1[r] = bload_nt<T>(s1, o); v.r1m2[r] = bload_nt<T>(s2, o);
                     v.r2m1[r] = bload_nt<T>(s3, o); v.r2m2[r] = bload_nt<T>(s4, o);
+                } else if constexpr (NTM == 2) {
+                    v.r1m1[r] = bload<T>(s1, o); v.r1m2[r] = bload_nt<T>(s2, o);
+                    v.r2m1[r] = bload<T>(s3, o); v.r2m2[r] = bload_nt<T>(s4, o);
                 } else {
                     v.r1m1[r] = bload<T>(s1, o); v.r1m2[r] = bload<T>(s2, o);
                     v.r2m1[r] = bload<T>(s3, o); v.r2m2[r] = bload<T>(s4, o);

@@ -199,7 +199,8 @@ int bd_westervelt_plan_rerun_step(bd_westervelt_plan* plan, void* stream);
  * the grid are Mur faces and the source plane. */
 long long bd_westervelt_plan_interior_cells(const bd_westervelt_plan* plan);
 /* Step-kernel launch shape chosen for the plan: tile columns, thread rows,
- * rows per thread, planes of loads in flight (0-3), non-temporal history loads (0/1) and
+ * rows per thread, planes of loads in flight (0-3), history-load cache policy (0 cached,
+ * 1 every level non-temporal, 2 only p^{n-3}, p^{n-4}, R^{n-2} non-temporal) and
  * workgroups per launch.  Any output pointer may be NULL.  Introspection
  * only; the reference has one fixed launch shape (W:560). */
 int bd_westervelt_plan_shape(const bd_westervelt_plan* plan, int* tile_x, int* tile_y, int* rows,

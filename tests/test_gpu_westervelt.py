@@ -309,6 +309,23 @@ def test_tile_shapes_fp32_bitwise_equal(monkeypatch, variant):
         assert np.abs(ea[:, k] - eb[:, k]).max() <= 1e-6 * np.abs(ea[:, k]).max()
 
 
+@pytest.mark.parametrize("nt", [0, 1, 2])
+@pytest.mark.parametrize("variant", [13, 10, 2])
+def test_history_cache_policies_bitwise_equal(monkeypatch, variant, nt):
+    # the history loads' cache policy (cached / every level non-temporal /
+    # only p^{n-3}, p^{n-4}, R^{n-2} non-temporal, forced with BD_WV_NT) does
+    # not touch the arithmetic: fields, pmax and focus equal the default's
+    n, nz, steps = 67, 40, 120
+    p, focus, _ = _tile_case(None, n, 0, nz, steps)
+    a = _run_shape(monkeypatch, variant, p, n, nz, steps, focus, torch.float32)
+    monkeypatch.setenv("BD_WV_NT", str(nt))
+    b = _run_shape(monkeypatch, variant, p, n, nz, steps, focus, torch.float32)
+    monkeypatch.delenv("BD_WV_NT", raising=False)
+    assert b.shape["nontemporal"] == nt and a.shape["nontemporal"] == 0
+    assert np.abs(host(a.pressure(0))).max() > 0
+    assert torch.equal(a.state, b.state) and torch.equal(a.pmax, b.pmax) and torch.equal(a.focus, b.focus)
+
+
 def test_default_shape_rule(monkeypatch):
     # without the measured choice of bd_westervelt_plan_prepare: one row per
     # thread with three planes of loads in flight for FP32, two for FP64; the
@@ -342,6 +359,7 @@ def test_prepare_chooses_a_measured_tile(monkeypatch, dtype):
     b = WesterveltRun(p, n, nz, dtype=dtype, max_steps=steps, focus=focus, prepare_graphs=False)
     sh = a.shape
     assert (sh["tile_x"], sh["tile_y"], sh["rows"], sh["prefetch"]) in [TILE_SHAPES[c] for c in cands]
+    assert sh["nontemporal"] in (0, 2)          # a state the last-level cache holds: cached or partial
     a.run(steps)
     b.run(steps)
     a.finalize(); b.finalize()
