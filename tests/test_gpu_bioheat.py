@@ -182,6 +182,33 @@ def test_every_triple_region_bitwise_equal(monkeypatch, variant, n, nz, kahan):
         assert torch.equal(a.TDc, b.TDc)
 
 
+@pytest.mark.parametrize("kahan", [True, False])
+def test_measured_choice_equals_single_substeps(monkeypatch, kahan):
+    # an FP32 call with at least 11 substeps left and no choice kept yet for
+    # its grid times a triple of each region shape and a pair (real substeps
+    # of the run) and keeps the fastest for the later calls: whatever it picks,
+    # the fields are the bits of the same substeps run one at a time
+    n, nz = 53, 23
+    p = two_layer(n, nz)
+    p.T_art = 310.15
+    p.perf = 2e-3
+    a = _random_state(p, n, nz, torch.float32, kahan, True, 5)
+    b = _random_state(p, n, nz, torch.float32, kahan, True, 5)
+    for k in ("BD_BH_FUSE", "BD_BH_VARIANT3"):
+        monkeypatch.delenv(k, raising=False)
+    for sx, sy, m in ((2, -1, 31), (0, 3, 17), (-4, 0, 5)):
+        monkeypatch.delenv("BD_BH_FUSE", raising=False)
+        a.step(sx, sy, m)
+        monkeypatch.setenv("BD_BH_FUSE", "0")
+        b.step(sx, sy, m)
+    monkeypatch.delenv("BD_BH_FUSE", raising=False)
+    torch.cuda.synchronize()
+    assert torch.equal(a.T[a.cur], b.T[b.cur])
+    assert torch.equal(a.TD, b.TD)
+    if kahan:
+        assert torch.equal(a.TDc, b.TDc)
+
+
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_differing_boundary_shells_paired_equals_single(monkeypatch, dtype):
     # the two fields start with different boundary shells: bd_bioheat_run
