@@ -311,19 +311,20 @@ constexpr int kDeepVariant = 2;      // 64 x 4 tile, two rows per thread, plane 
 constexpr int kDefaultBlocks = 1024;
 // Tiles bd_westervelt_plan_prepare times on the bound state and chooses from
 // (the first entry is the static choice of launch_shape's rule).  One MI355X,
-// FP32 tone burst, us per step, variants 2 / 10 / 13 (64 x 4 tile; two rows,
-// one plane in flight / one row, two planes / one row, three planes): 128^3
-// 29.0 / 24.7 / 24.3, 160^3 43.9 / 45.7 / 44.0, 192^3 71.4 / 68.0 / 71.4,
-// 224^3 122.6 / 125.7 / 122.7, 256^3 166.0 / 167.0 / 166.0; FP64 192^3
-// variants 0 / 10 / 13: 158.3 / 149.4 / 159.6.  No geometric rule separates
-// them, so the plan measures.
-constexpr int kTuneF32[] = {13, 10, kDeepVariant};
-constexpr int kTuneF64[] = {10, kDefaultVariant};
+// FP32 tone burst, us per step, variants 13 / 10 / 2 / 0 (64 x 4 tile; one
+// row, three planes of loads in flight / one row, two / two rows, one / one
+// row, none) with p^{n-3}, p^{n-4}, R^{n-2} non-temporal: 128^3 28.3 / 25.7 /
+// 32.0 / 27.0, 160^3 53.0 / 50.2 / 49.7 / -, 192^3 69.7 / 66.7 / 68.4 / 71.3,
+// 256^3 194.0 / 192.1 / 187.2 / -.  FP64 with the plan's cache policy,
+// variants 0 / 10: 41.0 / 53.7 (128^3), 134.3 / 147.3 (192^3), 363.7 / 399.1
+// (256^3).  No geometric rule separates the FP32 tiles, so the plan measures.
+constexpr int kTuneF32[] = {13, 10, kDeepVariant, kDefaultVariant};
+constexpr int kTuneF64[] = {kDefaultVariant, 10};
 // The tile taken without a measurement (plans without prepare, the
 // adjoint's forward recompute): one row per thread with three planes of
-// loads in flight for FP32 (never more than 5% off the fastest above, 16%
-// ahead at 128^3), two for FP64.
-constexpr int kStaticF32 = 13, kStaticF64 = 10;
+// loads in flight for FP32 (within 11% of the fastest above), one row without
+// prefetch for FP64.
+constexpr int kStaticF32 = 13, kStaticF64 = kDefaultVariant;
 
 // The 12-field state does not fit in the last-level cache: the step kernel
 // then loads the history levels with the non-temporal policy.

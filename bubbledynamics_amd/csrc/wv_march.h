@@ -143,8 +143,11 @@ struct PlaneIn {
     T r1m1[RY], r1m2[RY], r2m1[RY], r2m2[RY], pmx[RY];
 };
 
+// The FP64 one-row tile without prefetch is held to 128 VGPRs (4 waves per
+// SIMD, 12 bytes per lane spilled outside the march; 130 VGPRs and 3 waves
+// otherwise): 192^3 158 -> 135 us per step together with the loads above.
 template <typename T, int TX, int TY, int RY, int PF, int NTM>
-__global__ void __launch_bounds__(TX * TY)
+__global__ void __launch_bounds__(TX * TY, sizeof(T) == 8 && RY == 1 && PF == 0 ? 4 : 1)
 wv_march(WvMarchArgs<T> a, int step_arg) {
     constexpr int NT = TX * TY;
     constexpr int ROWS = TY * RY;
@@ -199,12 +202,59 @@ wv_march(WvMarchArgs<T> a, int step_arg) {
 
     constexpr uint32_t ES = sizeof(T);
     const uint32_t pbytes = (uint32_t)(plane * ES);
+    // Loads of plane z.  FP64 without prefetch: every load is issued whatever
+    // the plane (a water plane's relaxation levels and the peak pressure
+    // outside a window step read through zero-sized buffer descriptors: zeros,
+    // no memory access), so the number of loads in flight does not depend on
+    // the path taken and the compiler's waits stay exact.  FP32 measured
+    // slower that way (71.1 -> 73.3 us per step at 192^3) and keeps the
+    // branches, as do the prefetching tiles, whose plane buffers are copied at
+    // the back-edge.
     auto load_plane = [&](int z, PlaneIn<T, RY>& v) {
         const size_t zo = (size_t)z * plane;
         const bool tis = z >= a.g.iz_tissue;
         // p from plane z to the end of the field (halo rows past the grid read
         // memory of the next plane or, past the field, zeros: never used)
         const rsrc_t rp = make_rsrc(a.p + zo, (size_t)(a.g.Nz - z) * plane * ES);
+        if constexpr (!F32 && PF == 0) {
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                const uint32_t o = cidx[r] * ES;
+                v.hl[r] = xL ? bload<T>(rp, o + (uint32_t)(dxl * (int)ES)) : T(0);
+                v.hr[r] = xR ? bload<T>(rp, o + 3 * ES) : T(0);
+            }
+            v.hb = yB ? bload<T>(rp, offB * ES) : T(0);
+            v.ht = yT ? bload<T>(rp, offT * ES) : T(0);
+            const rsrc_t q1 = make_rsrc(a.pm1 + zo, pbytes), q2 = make_rsrc(a.pm2 + zo, pbytes);
+            const rsrc_t q3 = make_rsrc(a.pm3 + zo, pbytes), q4 = make_rsrc(a.pm4 + zo, pbytes);
+            const uint32_t rb = tis ? pbytes : 0u;
+            const rsrc_t s1 = make_rsrc(a.r1m1 + zo, rb), s2 = make_rsrc(a.r1m2 + zo, rb);
+            const rsrc_t s3 = make_rsrc(a.r2m1 + zo, rb), s4 = make_rsrc(a.r2m2 + zo, rb);
+            const rsrc_t sm = make_rsrc(pm_upd ? a.pmax + zo : a.p, pm_upd ? pbytes : 0u);
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                const uint32_t o = cidx[r] * ES;
+                v.qn[r] = bload<T>(rp, o, 3 * pbytes);
+                if constexpr (NTM == 1) {
+                    v.pm1[r] = bload_nt<T>(q1, o); v.pm2[r] = bload_nt<T>(q2, o);
+                    v.pm3[r] = bload_nt<T>(q3, o); v.pm4[r] = bload_nt<T>(q4, o);
+                    v.r1m1[r] = bload_nt<T>(s1, o); v.r1m2[r] = bload_nt<T>(s2, o);
+                    v.r2m1[r] = bload_nt<T>(s3, o); v.r2m2[r] = bload_nt<T>(s4, o);
+                } else if constexpr (NTM == 2) {
+                    v.pm1[r] = bload<T>(q1, o); v.pm2[r] = bload<T>(q2, o);
+                    v.pm3[r] = bload_nt<T>(q3, o); v.pm4[r] = bload_nt<T>(q4, o);
+                    v.r1m1[r] = bload<T>(s1, o); v.r1m2[r] = bload_nt<T>(s2, o);
+                    v.r2m1[r] = bload<T>(s3, o); v.r2m2[r] = bload_nt<T>(s4, o);
+                } else {
+                    v.pm1[r] = bload<T>(q1, o); v.pm2[r] = bload<T>(q2, o);
+                    v.pm3[r] = bload<T>(q3, o); v.pm4[r] = bload<T>(q4, o);
+                    v.r1m1[r] = bload<T>(s1, o); v.r1m2[r] = bload<T>(s2, o);
+                    v.r2m1[r] = bload<T>(s3, o); v.r2m2[r] = bload<T>(s4, o);
+                }
+                v.pmx[r] = bload<T>(sm, o);
+            }
+            return;
+        }
         const rsrc_t q1 = make_rsrc(a.pm1 + zo, pbytes), q2 = make_rsrc(a.pm2 + zo, pbytes);
         const rsrc_t q3 = make_rsrc(a.pm3 + zo, pbytes), q4 = make_rsrc(a.pm4 + zo, pbytes);
 #pragma unroll

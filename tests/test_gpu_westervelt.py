@@ -328,12 +328,12 @@ def test_history_cache_policies_bitwise_equal(monkeypatch, variant, nt):
 
 def test_default_shape_rule(monkeypatch):
     # without the measured choice of bd_westervelt_plan_prepare: one row per
-    # thread with three planes of loads in flight for FP32, two for FP64; the
+    # thread with three planes of loads in flight for FP32, none for FP64; the
     # history loads are non-temporal once the 12-field state exceeds 256 MB
     monkeypatch.delenv("BD_WV_VARIANT", raising=False)
     monkeypatch.delenv("BD_WV_BLOCKS", raising=False)
     expect = {(192, torch.float32): (1, 3, 1), (128, torch.float32): (1, 3, 0),
-              (160, torch.float32): (1, 3, 0), (192, torch.float64): (1, 2, 1)}
+              (160, torch.float32): (1, 3, 0), (192, torch.float64): (1, 0, 1)}
     for (n, dt), (rows, pf, nt) in expect.items():
         p = presets.tone_burst_192(n)
         run = WesterveltRun(p, n, n, dtype=dt, max_steps=1, pmax=False, energy=False, prepare_graphs=False)
@@ -352,7 +352,7 @@ def test_prepare_chooses_a_measured_tile(monkeypatch, dtype):
     # the static tile bitwise (energy to the partials' grouping)
     monkeypatch.delenv("BD_WV_VARIANT", raising=False)
     monkeypatch.delenv("BD_WV_BLOCKS", raising=False)
-    cands = {torch.float32: (13, 10, 2), torch.float64: (10, 0)}[dtype]
+    cands = {torch.float32: (13, 10, 2, 0), torch.float64: (0, 10)}[dtype]
     n, nz, steps = 67, 40, 60
     p, focus, _ = _tile_case(None, n, 0, nz, steps)
     a = WesterveltRun(p, n, nz, dtype=dtype, max_steps=steps, focus=focus)             # measured tile
