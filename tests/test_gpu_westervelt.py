@@ -310,16 +310,19 @@ def test_tile_shapes_fp32_bitwise_equal(monkeypatch, variant):
 
 
 @pytest.mark.parametrize("nt", [0, 1, 2])
-@pytest.mark.parametrize("variant", [13, 10, 2])
-def test_history_cache_policies_bitwise_equal(monkeypatch, variant, nt):
+@pytest.mark.parametrize("variant,dtype", [(13, torch.float32), (10, torch.float32), (2, torch.float32),
+                                           (0, torch.float32), (0, torch.float64), (10, torch.float64)])
+def test_history_cache_policies_bitwise_equal(monkeypatch, variant, dtype, nt):
     # the history loads' cache policy (cached / every level non-temporal /
     # only p^{n-3}, p^{n-4}, R^{n-2} non-temporal, forced with BD_WV_NT) does
     # not touch the arithmetic: fields, pmax and focus equal the default's
+    # (FP64 variant 0 issues every load of a plane, through zero-sized
+    # descriptors where the branchy tiles skip them)
     n, nz, steps = 67, 40, 120
     p, focus, _ = _tile_case(None, n, 0, nz, steps)
-    a = _run_shape(monkeypatch, variant, p, n, nz, steps, focus, torch.float32)
+    a = _run_shape(monkeypatch, variant, p, n, nz, steps, focus, dtype)
     monkeypatch.setenv("BD_WV_NT", str(nt))
-    b = _run_shape(monkeypatch, variant, p, n, nz, steps, focus, torch.float32)
+    b = _run_shape(monkeypatch, variant, p, n, nz, steps, focus, dtype)
     monkeypatch.delenv("BD_WV_NT", raising=False)
     assert b.shape["nontemporal"] == nt and a.shape["nontemporal"] == 0
     assert np.abs(host(a.pressure(0))).max() > 0
