@@ -57,6 +57,29 @@ def build_library(verbose: bool = False) -> Path:
     return LIB
 
 
+HOST_DIR = PKG / "host"
+HOST_PROGRAMS = ("gilmore_threshold", "westervelt_dropin")
+
+
+def build_host(verbose: bool = False) -> list[Path]:
+    """C host programs over the C ABI (bubbledynamics_amd/host/*.c), linked
+    against the in-tree library: bd_<name> next to their sources."""
+    rocm = Path(HIPCC).resolve().parent.parent
+    out = []
+    for name in HOST_PROGRAMS:
+        exe = HOST_DIR / f"bd_{name}"
+        tmp = exe.with_suffix(".tmp")
+        _run([os.environ.get("CC", "gcc"), "-std=c11", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__",
+              f"-I{rocm / 'include'}", f"-I{ROOT / 'include'}", str(HOST_DIR / f"{name}.c"), "-o", str(tmp),
+              f"-L{PKG}", "-lbubbledynamics", f"-L{rocm / 'lib'}", "-lamdhip64", "-lm",
+              "-Wl,-rpath,$ORIGIN/..", f"-Wl,-rpath,{rocm / 'lib'}"])
+        os.replace(tmp, exe)
+        out.append(exe)
+        if verbose:
+            print(f"built {exe}")
+    return out
+
+
 def build_oracle(verbose: bool = False) -> Path:
     srcs = sorted(ORACLE_DIR.glob("*.cpp"))
     tmp = ORACLE_LIB.with_suffix(".so.tmp")
@@ -70,3 +93,4 @@ def build_oracle(verbose: bool = False) -> Path:
 if __name__ == "__main__":
     build_oracle(verbose=True)
     build_library(verbose=True)
+    build_host(verbose=True)

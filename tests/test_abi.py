@@ -43,3 +43,15 @@ def test_struct_sizes_match_c_layout():
     assert C.sizeof(WesterveltParams) == 25 * 8 + 4 * 4 + 3 * 8 + 2 * 4 + 3 * 8 + 8 * 8
     assert C.sizeof(GilmoreParams) == 11 * 8 + 2 * 4 + 4 * 8 + 4 * 8 + 7 * 8
     assert C.sizeof(GilmoreBatch) == 8 * 4 + 3 * 8 + 5 * 8 + 8 + 8 * 8 + 4 * 8
+
+
+def test_c_host_programs_link_and_print_usage():
+    # the C host programs link against the library and the HIP runtime and
+    # start without a GPU (usage text, exit status 2)
+    import subprocess
+    host = ROOT / "bubbledynamics_amd" / "host"
+    for name in ("gilmore_threshold", "westervelt_dropin"):
+        exe = host / f"bd_{name}"
+        assert exe.exists(), f"{exe} is not built (python -m bubbledynamics_amd.build)"
+        r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2 and "usage:" in r.stderr, (name, r.returncode, r.stderr)
