@@ -58,7 +58,7 @@ def build_library(verbose: bool = False) -> Path:
 
 
 HOST_DIR = PKG / "host"
-HOST_PROGRAMS = ("gilmore_threshold", "westervelt_dropin")
+HOST_PROGRAMS = ("gilmore_threshold", "westervelt_dropin", "temperature_spiral")
 
 
 def build_host(verbose: bool = False) -> list[Path]:

@@ -50,7 +50,7 @@ def test_c_host_programs_link_and_print_usage():
     # start without a GPU (usage text, exit status 2)
     import subprocess
     host = ROOT / "bubbledynamics_amd" / "host"
-    for name in ("gilmore_threshold", "westervelt_dropin"):
+    for name in ("gilmore_threshold", "westervelt_dropin", "temperature_spiral"):
         exe = host / f"bd_{name}"
         assert exe.exists(), f"{exe} is not built (python -m bubbledynamics_amd.build)"
         r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)

@@ -60,3 +60,23 @@ def test_c_threshold_table_equals_python_sweep(tmp_path):
     assert a[:-1] == b[:-1]                    # header and every row; the last line is the run time
     assert a[-1].startswith("time")
     torch.cuda.synchronize()
+
+
+def test_c_thermal_program_drop_in_against_run(tmp_path):
+    # the reference thermal program's flow in C on a 101 x 101 x 121 grid:
+    # Q handed over through a Q3D file with the mirror expansion, initial
+    # state by region, the spiral's first 400 shifts with one drop-in
+    # Temperature_square per time step, the end-of-run slices; then the same
+    # substeps through bd_bioheat_run (runs of equal shift, pairs per launch)
+    r = _run("temperature_spiral", 101, 121, 400, tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    print(r.stdout.strip())
+    m = re.search(r"max\|T\| (\S+) max\|dT\| (\S+) max\|TD\| (\S+) max\|dTD\| (\S+) finite (\d)", r.stdout)
+    tmax, dT, dmax, dTD = (float(m.group(k)) for k in range(1, 5))
+    assert m.group(5) == "1"
+    assert tmax > 273.15 + 23.6 + 1.0               # the focus heated by more than 1 K
+    assert dT <= 1e-12 * tmax and dTD <= 1e-12 * dmax
+    for name in ("Q3D_spiral.bin", "T_xz.txt", "TD_xz.txt"):
+        assert (tmp_path / name).stat().st_size > 0
+    rows = (tmp_path / "T_xz.txt").read_text().splitlines()
+    assert rows[0].startswith("Z=") and len(rows) == 1 + 121
