@@ -371,8 +371,12 @@ struct GmBatchArgs {
     int resume;
 };
 
-template <int METHOD, int PREC, int ARITH>
-__global__ void __launch_bounds__(256) gm_batch_kernel(GmBatchArgs a) {
+// MINW: wavefronts per SIMD the register allocation must allow (the
+// DP5(4) kernels in the reference arithmetic and in the split take more than
+// 256 registers and run one wavefront per SIMD; asked for two, they spill to
+// scratch: faster while every lane has work, slower in the tail).
+template <int METHOD, int PREC, int ARITH, int MINW = 1>
+__global__ void __launch_bounds__(256, MINW) gm_batch_kernel(GmBatchArgs a) {
     static_assert(ARITH == BD_GM_ARITH_FAST || PREC == BD_GM_FP64, "reference arithmetic is FP64 only");
     const GmConsts& K = a.K;
     int next = 0, end = 0;          // this lane's current chunk of bubble indices
@@ -1189,6 +1193,9 @@ struct GmPass {                  // order / slicing of one launch of the batch k
     long long* cost = nullptr;
     long long slice = 0;
     int resume = 0;
+    // the DP5(4) kernels of the reference arithmetic and of the split in
+    // their build for two wavefronts per SIMD (same results)
+    int two_waves = 0;
 };
 
 static int gm_launch(const bd_gilmore_params* prm, const bd_gilmore_batch* b, void* workspace, GmShape shp,
@@ -1233,10 +1240,14 @@ static int gm_launch(const bd_gilmore_params* prm, const bd_gilmore_batch* b, vo
         hipLaunchKernelGGL((gm_batch_kernel<BD_GM_RK4, BD_GM_FP64, FAST>), grid, blk, 0, st, a);
     else if (b->method == BD_GM_RK4)
         hipLaunchKernelGGL((gm_batch_kernel<BD_GM_RK4, BD_GM_SPLIT, FAST>), grid, blk, 0, st, a);
+    else if (b->precision == BD_GM_FP64 && ref && pass.two_waves)
+        hipLaunchKernelGGL((gm_batch_kernel<BD_GM_DP45, BD_GM_FP64, REF, 2>), grid, blk, 0, st, a);
     else if (b->precision == BD_GM_FP64 && ref)
         hipLaunchKernelGGL((gm_batch_kernel<BD_GM_DP45, BD_GM_FP64, REF>), grid, blk, 0, st, a);
     else if (b->precision == BD_GM_FP64)
         hipLaunchKernelGGL((gm_batch_kernel<BD_GM_DP45, BD_GM_FP64, FAST>), grid, blk, 0, st, a);
+    else if (pass.two_waves)
+        hipLaunchKernelGGL((gm_batch_kernel<BD_GM_DP45, BD_GM_SPLIT, FAST, 2>), grid, blk, 0, st, a);
     else
         hipLaunchKernelGGL((gm_batch_kernel<BD_GM_DP45, BD_GM_SPLIT, FAST>), grid, blk, 0, st, a);
     BD_LAUNCH_CHECK();
@@ -1272,6 +1283,12 @@ extern "C" int bd_gilmore_integrate(const bd_gilmore_params* prm, const bd_gilmo
     // Passes of growing slices (S, 16 S, then to the end): every pass runs the
     // bubbles still unfinished, largest predicted remaining work first, so a
     // long bubble the previous ranking underrated is moved up after one slice.
+    // The two sliced passes keep every lane busy and take the kernels built
+    // for two wavefronts per SIMD; the last pass ends in a tail of a few long
+    // bubbles, where a wavefront's own speed counts, and takes the builds that
+    // do not spill (1e6 bubbles, 20 us, reference arithmetic: passes of
+    // 72 / 756 / 1,381 ms with one wavefront per SIMD, 59 / 633 / 1,565 ms
+    // with two).
     GmSched sc;
     if (!gm_sched_scratch(b->n, &sc)) return BD_ERR_ALLOC;
     const long long S = b->slice > 0 ? b->slice : 128;
@@ -1287,6 +1304,7 @@ extern "C" int bd_gilmore_integrate(const bd_gilmore_params* prm, const bd_gilmo
         pp.cost = sc.cost;
         pp.slice = slices[pass];
         pp.resume = pass > 0;
+        pp.two_waves = slices[pass] != 0;
         rc = gm_launch(prm, &bp, workspace, shp, st, pp);
         if (rc) return rc;
         if (slices[pass] == 0) break;

@@ -138,26 +138,30 @@ def test_split_precision_rmax_error(orc):
     T = 20e-6
     b = gm.BubbleBatch(R0, amp, ph, fr)
     o64 = b.run(p, BD_GM_DP45, T, 1e-13, rtol=1e-8, atol=1e-12)
-    o32 = b.run(p, BD_GM_DP45, T, 1e-13, precision=BD_GM_SPLIT, rtol=1e-8, atol=1e-12)
-    torch.cuda.synchronize()
     ref = orc.gilmore_integrate(p, BD_GM_DP45, R0, amp, ph, fr, T, 1e-13, rtol=1e-8, atol=1e-12, nthreads=16)
     fix = json.loads((Path(__file__).parent / "golden" / "gilmore_conditioning.json").read_text())
     ill = set(fix["config3_subset_ulp"]["positions"])
     well = np.array([i not in ill for i in range(len(idx))])
-    rel = np.abs(host(o32["Rmax"]) / ref["Rmax"] - 1)
-    rel64 = np.abs(host(o32["Rmax"]) / host(o64["Rmax"]) - 1)
-    flips = host(o32["t_collapse"] > 0) != (ref["t_collapse"] > 0)
-    print(f"split vs oracle: median {np.median(rel):.3e} p99 {np.quantile(rel, 0.99):.3e} max {rel.max():.3e}; "
-          f"well-conditioned max {rel[well].max():.3e}, > 1e-6: {int(np.sum(rel[well] > 1e-6))}/{well.sum()}; "
-          f"vs all-FP64 median {np.median(rel64):.3e}; collapse flips {int(flips.sum())}; accepted steps "
-          f"split {host(o32['n_acc']).sum()} FP64 {host(o64['n_acc']).sum()}")
-    keep = well & ~flips
-    print(f"split: well-conditioned bubbles without a collapse flip: max {rel[keep].max():.3e}, "
-          f"p99 {np.quantile(rel[keep], 0.99):.3e}")
-    # measured: median 3.3e-6, 14 flips in 4,096; the bounds guard that level
-    assert np.median(rel) <= 1e-5
-    assert np.quantile(rel[keep], 0.99) <= 1e-2
-    assert np.mean(flips) <= 0.01
+    # input order, and the sliced schedule (its first two passes run the
+    # kernel's build for two wavefronts per SIMD)
+    for schedule in (gm.BD_GM_SCHED_INPUT, gm.BD_GM_SCHED_SLICED):
+        o32 = b.run(p, BD_GM_DP45, T, 1e-13, precision=BD_GM_SPLIT, rtol=1e-8, atol=1e-12, schedule=schedule)
+        torch.cuda.synchronize()
+        rel = np.abs(host(o32["Rmax"]) / ref["Rmax"] - 1)
+        rel64 = np.abs(host(o32["Rmax"]) / host(o64["Rmax"]) - 1)
+        flips = host(o32["t_collapse"] > 0) != (ref["t_collapse"] > 0)
+        print(f"split (schedule {schedule}) vs oracle: median {np.median(rel):.3e} p99 {np.quantile(rel, 0.99):.3e} "
+              f"max {rel.max():.3e}; well-conditioned max {rel[well].max():.3e}, > 1e-6: "
+              f"{int(np.sum(rel[well] > 1e-6))}/{well.sum()}; vs all-FP64 median {np.median(rel64):.3e}; collapse "
+              f"flips {int(flips.sum())}; accepted steps split {host(o32['n_acc']).sum()} FP64 "
+              f"{host(o64['n_acc']).sum()}")
+        keep = well & ~flips
+        print(f"split: well-conditioned bubbles without a collapse flip: max {rel[keep].max():.3e}, "
+              f"p99 {np.quantile(rel[keep], 0.99):.3e}")
+        # measured: median 3.3e-6, 14 flips in 4,096; the bounds guard that level
+        assert np.median(rel) <= 1e-5
+        assert np.quantile(rel[keep], 0.99) <= 1e-2
+        assert np.mean(flips) <= 0.01
 
 
 def test_queue_scheduling_does_not_change_results():
@@ -193,12 +197,16 @@ def test_autotune_picks_a_candidate_and_shape_does_not_change_results():
     assert torch.equal(tuned["n_rej"], base["n_rej"])
 
 
-@pytest.mark.parametrize("method", [BD_GM_DP45, BD_GM_RK4])
-def test_sliced_schedule_gives_the_same_results(method):
+@pytest.mark.parametrize("method,arith", [(BD_GM_DP45, "fast"), (BD_GM_DP45, "reference"), (BD_GM_RK4, "fast")],
+                         ids=["dp45", "dp45-reference", "rk4"])
+def test_sliced_schedule_gives_the_same_results(method, arith):
     # slicing, suspending and the cost-ordered second pass change only which
-    # lane takes which bubble when: every output matches the input order bitwise
-    from bubbledynamics_amd._ffi import BD_GM_SCHED_SLICED
-    p = gm.workload_params()
+    # lane takes which bubble when (and, for the reference arithmetic, which
+    # register build of the kernel runs a pass): every output matches the
+    # input order bitwise.  (The split is left out: it chooses FP32 or FP64
+    # per wavefront, so its results depend on which bubbles share one.)
+    from bubbledynamics_amd._ffi import BD_GM_ARITH_FAST, BD_GM_ARITH_REFERENCE, BD_GM_SCHED_SLICED
+    p = gm.workload_params(arithmetic=BD_GM_ARITH_REFERENCE if arith == "reference" else BD_GM_ARITH_FAST)
     R0, amp, ph, fr = gm.config3_inputs(6000, 3)
     b = gm.BubbleBatch(R0, amp, ph, fr)
     dt = 1e-13 if method == BD_GM_DP45 else 1e-10
