@@ -806,6 +806,9 @@ def main() -> int:
     for _ in range(5):
         _ffi.check("rerun_step", lib.bd_westervelt_plan_rerun_step(run32._plan, C.c_void_p(stream)))
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # the relaunches queue behind a short device-side wait (a few ms at most),
+    # so that they run back to back whatever the host's launch rate
+    torch.cuda._sleep(5_000_000)
     e0.record()
     for _ in range(R):
         _ffi.check("rerun_step", lib.bd_westervelt_plan_rerun_step(run32._plan, C.c_void_p(stream)))
