@@ -372,8 +372,10 @@ def bioheat_shapes_figures() -> dict:
     """FP32 Pennes substeps with the compensated dose at several grid widths:
     pairs against the two region shapes of the three-substep kernel (forced
     through BD_BH_FUSE / BD_BH_VARIANT3, which the library reads on every
-    call), and the choice bd_bioheat_run measures by itself; us per substep,
-    best of three runs of 60 substeps after a 6-substep warm-up."""
+    call), and the choice bd_bioheat_run measures by itself; then FP64 pairs
+    as the reference-precision thermal program runs them (plain dose, stored
+    temperatures); us per substep, best of three runs of 60 substeps after a
+    6-substep warm-up."""
     import torch
     from bubbledynamics_amd import bioheat as bh
     p = bh.pennes_params(dx=2e-4, dt=0.05, substeps=2)
@@ -383,11 +385,15 @@ def bioheat_shapes_figures() -> dict:
         Q[nz // 2, n // 2, n // 2] = 1e9
         row = {}
         for name, env in (("pairs", {"BD_BH_FUSE": "2"}), ("triples_64x32", {"BD_BH_VARIANT3": "0"}),
-                          ("triples_128x16", {"BD_BH_VARIANT3": "1"}), ("measured_choice", {})):
+                          ("triples_128x16", {"BD_BH_VARIANT3": "1"}), ("measured_choice", {}), ("pairs_f64", {})):
             saved = {k: os.environ.pop(k, None) for k in ("BD_BH_FUSE", "BD_BH_VARIANT3")}
             os.environ.update(env)
             try:
-                run = bh.BioheatRun(p, n, nz, Q, p.T_art, dtype=torch.float32, kahan=True, offset=True)
+                if name.endswith("_f64"):
+                    run = bh.BioheatRun(p, n, nz, Q.double(), p.T_art, dtype=torch.float64, kahan=False,
+                                        offset=False)
+                else:
+                    run = bh.BioheatRun(p, n, nz, Q, p.T_art, dtype=torch.float32, kahan=True, offset=True)
                 run.step(0, 0, 6)
                 torch.cuda.synchronize()
                 best = 1e9
